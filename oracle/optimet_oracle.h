@@ -1,0 +1,111 @@
+/*
+ * optimet_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU FP64 restatement of the OPTIMET srcAna hot path (analytic spheres):
+ * spherical Bessel/Hankel functions, spherical harmonics, the Stout (2002)
+ * translation-addition recurrence, the A/B coupling blocks, Mie T-matrices,
+ * the plane-wave source, the preconditioned scattering matrix S = I - T.A
+ * (and its second-harmonic twin V), the dense solves, the second-harmonic
+ * source vector with its Clebsch-Gordan / Wigner tables, and the cross
+ * sections.  Only tests/, the smoke check and the CPU-baseline leg of the
+ * benchmark may load this library; the product path never does.
+ *
+ * Complex numbers cross this interface as interleaved (re, im) doubles.
+ * Every function returns 0 on success and a negative code on error; the text
+ * of the last error is available from ora_last_error().
+ */
+#ifndef OPTIMET_ORACLE_H
+#define OPTIMET_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One cluster-solve problem, in the units the reference's reader produces
+ * (Reader.cpp:593-605, 782-831): lengths in metres, angles in radians,
+ * permittivities/permeabilities relative to vacuum. */
+typedef struct {
+  int nobj;
+  int nmax;
+  int sh;                 /* second-harmonic stage requested */
+  const double *xyz;      /* 3*nobj Cartesian centres */
+  const double *radius;   /* nobj */
+  const double *eps_r;    /* 2*nobj complex, at omega */
+  const double *mu_r;     /* 2*nobj complex */
+  const double *eps_r_sh; /* 2*nobj complex, at 2*omega */
+  const double *ksippp;   /* 2*nobj complex, chi_perp-perp-perp */
+  const double *ksiparppar; /* 2*nobj complex, chi_par-perp-par */
+  const double *gamma;    /* 2*nobj complex, bulk coefficient */
+  double bg_eps_r[2];
+  double bg_mu_r[2];
+  double lambda;          /* vacuum wavelength, m */
+  double theta_k, phi_k;  /* propagation direction */
+  double etheta[2], ephi[2]; /* polarisation */
+} ora_problem;
+
+const char *ora_last_error(void);
+
+/* type 0: spherical Bessel j_n; type 1: spherical Hankel h_n^(1).
+ * data, ddata: (nmax+1) complex each. */
+int ora_bessel(int type, double zr, double zi, int nmax, double *data, double *ddata);
+
+/* Y_n^m(theta, phi) with the Condon-Shortley phase. */
+int ora_ynm(int n, int m, double theta, double phi, double *out);
+
+/* Translation-addition coefficient C(n,m,l,k) of a fresh
+ * TranslationAdditionCoefficients(R, waveK, regular) instance. */
+int ora_tac(const double *R_sph, double kr, double ki, int regular, int n, int m, int l, int k,
+            double *out);
+
+/* Coupling(R, waveK, nmax, regular): A ("diagonal") and B ("offdiagonal"),
+ * each nf*nf complex, row-major A[p*nf+q] with nf = nmax(nmax+2). */
+int ora_coupling(const double *R_sph, double kr, double ki, int nmax, int regular, double *A,
+                 double *B);
+
+/* Per-object diagonal vectors, 2*nf complex each (TE block first).
+ * which: 0 T (getTLocal), 1 T_SH (getTLocalSH), 2 Iaux, 3 IauxSH1, 4 IauxSH2,
+ *        5 TSH1_outer, 6 TSH2_outer */
+int ora_object_vector(const ora_problem *pb, int obj, int which, double *out);
+
+/* Plane-wave coefficients dataIncAp, dataIncBp: nf complex each. */
+int ora_incident(const ora_problem *pb, double *ap, double *bp);
+
+/* Excitation::getIncLocal at object obj: 2*nf complex. */
+int ora_inc_local(const ora_problem *pb, int obj, double *out);
+
+/* Q = source_vector(): N complex. */
+int ora_source_vector(const ora_problem *pb, double *Q);
+
+/* S (sh=0) or V (sh=1): N*N complex, column-major. */
+int ora_scattering_matrix(const ora_problem *pb, int sh, double *S);
+
+/* Dense solve of S x = b (column-major, N*N). method 0: LU with partial
+ * pivoting; 1: column-pivoted Householder QR. */
+int ora_dense_solve(int N, const double *S, const double *b, double *x, int method);
+
+/* The nine coupling tables of Geometry::Coefficients, each nfs*nf*nf doubles,
+ * order C_10m1, C_11m1, C_00m1, C_01m1, W_m1m1, W_11, W_00, W_10, W_01. */
+int ora_cg_tables(int nmax, int nmaxs, double *tables);
+
+/* Wigner symbols (for pinning against exact rationals). */
+double ora_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3);
+double ora_wigner6j(int j1, int j2, int j3, int j4, int j5, int j6);
+double ora_wigner9j(int a, int b, int c, int d, int e, int f, int g, int h, int i);
+
+/* source_vectorSH and source_vectorSH_K1ana from conj(x_int): N complex each. */
+int ora_source_sh(const ora_problem *pb, const double *tables, const double *x_int, double *K,
+                  double *K1ana);
+
+/* Full PreconditionedMatrix update()+solve(): x_sca, x_int (N complex each) and,
+ * if pb->sh, x_sca_sh, x_int_sh.  xs: [Cext, Csca, Cabs, Csca_SH]. */
+int ora_solve_all(const ora_problem *pb, int method, double *x_sca, double *x_int,
+                  double *x_sca_sh, double *x_int_sh, double *xs);
+
+/* Cross sections from solved coefficients: xs = [Cext, Csca, Cabs, Csca_SH]. */
+int ora_cross_sections(const ora_problem *pb, const double *x_sca, const double *x_sca_sh,
+                       double *xs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
