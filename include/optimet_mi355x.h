@@ -1,0 +1,140 @@
+/*
+ * optimet_mi355x.h -- C ABI of the MI355X multiple-scattering backend.
+ *
+ * The library sits behind OPTIMET's solver interface
+ * optimet::solver::AbstractSolver (srcAna/Solver.h:62-144):
+ *   omx_set_geometry / omx_set_excitation  <- the Geometry and Excitation
+ *       objects handed to the solver's constructor and to
+ *       AbstractSolver::update(geometry, incWave) (Solver.h:70-74, 94-99);
+ *   omx_update                              <- AbstractSolver::update()
+ *       (Solver.h:103; PreconditionedMatrixSolver.h:82-100);
+ *   omx_solve                               <- AbstractSolver::solve(X_sca, X_int,
+ *       X_sca_SH, X_int_SH, CGcoeff) (Solver.h:86-87; PreconditionedMatrixSolver.h:45-79);
+ *   omx_set_cg_tables                       <- the CGcoeff argument of solve()
+ *       (the nine tables of Geometry::Coefficients, Geometry.cpp:234-247);
+ *   omx_cross_sections                      <- Result::getExtinctionCrossSection,
+ *       getScatteringCrossSection, getAbsorptionCrossSection and
+ *       getScatteringCrossSection_SH (Result.cpp:557-761);
+ *   omx_scattering_size                     <- AbstractSolver::scattering_size()
+ *       (Solver.h:132-135).
+ *
+ * Conventions (the reference's, srcAna/Types.in.h:46-52, Reader.cpp:593-831):
+ * complex numbers are interleaved (re, im) doubles; coefficient vectors hold,
+ * per object, nf = nmax(nmax+2) TE ("b_n") coefficients followed by nf TM
+ * ("a_n") ones, harmonic index p = n(n+1) - m - 1; lengths in metres, angles
+ * in radians; permittivities and permeabilities relative to vacuum.
+ *
+ * Errors: every function returns OMX_OK (0) or a negative OMX_ERR_* code and
+ * never throws; omx_last_error() returns the calling thread's last message.
+ * A context is used by one host thread at a time; distinct contexts are
+ * independent.
+ */
+#ifndef OPTIMET_MI355X_H
+#define OPTIMET_MI355X_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OMX_OK 0
+#define OMX_ERR_ARG -1       /* invalid argument or unsupported size */
+#define OMX_ERR_DEVICE -2    /* HIP runtime error (no device, launch failure) */
+#define OMX_ERR_MEMORY -3    /* device allocation failed */
+#define OMX_ERR_SINGULAR -4  /* zero pivot in the LU factorisation */
+#define OMX_ERR_STATE -5     /* call out of order (e.g. solve before update) */
+#define OMX_ERR_NOCONV -6    /* iterative solver did not reach its tolerance */
+#define OMX_ERR_OVERLAP -7   /* overlapping spheres (Geometry.cpp:39-51) */
+
+/* Solver kinds: the backend selected from the XML input. */
+#define OMX_SOLVER_LU64 0     /* complex FP64 LU with partial pivoting */
+#define OMX_SOLVER_LU32_IR 1  /* complex FP32 LU on MFMA + FP64 GMRES-IR */
+#define OMX_SOLVER_GMRES64 2  /* restarted GMRES, FP64 matrix streamed from HBM */
+
+typedef struct omx_context omx_context;
+
+typedef struct {
+  int N;                 /* unknowns per frequency, 2 P nf */
+  int Np;                /* padded matrix order */
+  int iterations;        /* GMRES / refinement inner iterations (fundamental) */
+  int iterations_sh;     /* same, second harmonic */
+  double residual;       /* final ||Q - S x|| / ||Q|| (fundamental) */
+  double residual_sh;    /* final ||K - V x|| / ||K|| (second harmonic) */
+  double t_update_ms;    /* last omx_update: assembly of Q, S (and V) */
+  double t_factor_ms;    /* last omx_solve: factorisations */
+  double t_solve_ms;     /* last omx_solve: triangular solves / iterations */
+  double t_sh_source_ms; /* last omx_solve: second-harmonic source vector */
+  double t_assemble_ms;  /* last omx_update: pair-coefficient kernels only */
+  size_t device_bytes;   /* device memory held by the context */
+} omx_stats;
+
+const char *omx_last_error(void);
+
+/* Context on HIP device `device`. */
+int omx_create(omx_context **ctx, int device);
+void omx_destroy(omx_context *ctx);
+
+/* Geometry: nobj spheres with Cartesian centres xyz[3*nobj] (Reader.cpp:593-599
+ * converts these to Scatterer::vR), radius[nobj], and per sphere the complex
+ * eps_r, mu_r, eps_r_sh (relative permittivity at 2 omega), ksippp (chi_perp-perp-perp),
+ * ksiparppar (chi_par-perp-par) and gamma (ElectroMagnetic::init_r,
+ * ElectroMagnetic.cpp:34-54); bg_eps_r, bg_mu_r describe the background.
+ * All spheres share nmax (PreconditionedMatrix.cpp:1162-1165); 1 <= nmax <= 10.
+ * eps_r_sh, ksippp, ksiparppar, gamma may be NULL when no second harmonic is solved. */
+int omx_set_geometry(omx_context *ctx, int nobj, int nmax, const double *xyz, const double *radius,
+                     const double *eps_r, const double *mu_r, const double *eps_r_sh,
+                     const double *ksippp, const double *ksiparppar, const double *gamma,
+                     const double *bg_eps_r, const double *bg_mu_r);
+
+/* Plane-wave excitation (Reader.cpp:782-831, Excitation.cpp:31-75): vacuum
+ * wavelength (m), propagation direction theta, phi (rad), polarisation
+ * E_theta, E_phi (complex), and the second-harmonic flag SH_cond. */
+int omx_set_excitation(omx_context *ctx, double wavelength, double theta, double phi,
+                       const double *etheta, const double *ephi, int sh);
+
+/* Solver selection; tol is the relative residual target of the iterative
+ * parts (GMRES-IR refinement, GMRES), maxit their iteration cap, restart the
+ * GMRES restart length. */
+int omx_set_solver(omx_context *ctx, int kind, double tol, int maxit, int restart);
+
+/* The nine Clebsch-Gordan / Wigner tables of the second-harmonic source
+ * (CGcoeff, Simulation.cpp:613-616), each nf*nf*nf doubles in the order
+ * C_10m1, C_11m1, C_00m1, C_01m1, W_m1m1, W_11, W_00, W_10, W_01, indexed
+ * [k][p][q].  Copied to the device; the pointers are not retained. */
+int omx_set_cg_tables(omx_context *ctx, int nmax, const double *const *tables);
+/* Builds the same nine tables on the device (Symbol.cpp:1036-1451). */
+int omx_build_cg_tables(omx_context *ctx, int nmax);
+
+/* AbstractSolver::update(): plane-wave coefficients, T-matrices, source
+ * vector Q, scattering matrix S and, with SH_cond, V on the device. */
+int omx_update(omx_context *ctx);
+
+/* AbstractSolver::solve(): writes N complex values into each non-NULL output
+ * (x_sh and x_int_sh only when SH_cond). */
+int omx_solve(omx_context *ctx, double *x_sca, double *x_int, double *x_sca_sh, double *x_int_sh);
+
+/* [C_ext, C_sca, C_abs, C_sca_SH] of the last solve, with the reference's
+ * conventions (the incoherent C_sca sum of Result.cpp:629-631 included). */
+int omx_cross_sections(omx_context *ctx, double *out4);
+
+/* 2 nmax (nmax+2) P, AbstractSolver::scattering_size(). */
+int omx_scattering_size(const omx_context *ctx);
+
+int omx_get_stats(const omx_context *ctx, omx_stats *out);
+
+/* Parity access: copies the assembled N x N matrix (which 0: S, 1: V; FP64,
+ * column-major) or a vector (which 0: Q, 1: K, 2: K1ana, 3: local incident
+ * field of every object) to host memory. */
+int omx_read_matrix(omx_context *ctx, int which, double *out);
+int omx_read_vector(omx_context *ctx, int which, double *out);
+
+/* Stand-alone dense complex solve A x = b on the device (A column-major,
+ * N x N); kind OMX_SOLVER_LU64 or OMX_SOLVER_LU32_IR.  Uses the context's
+ * device and stream. */
+int omx_dense_solve(omx_context *ctx, int N, const double *A, const double *b, double *x, int kind);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,133 @@
+// omx_iter.hip -- level-1 kernels of the iterative solvers (GMRES, GMRES-IR)
+// and the element-wise unpreconditioning steps (Solver.cpp:57-116).
+#include "omx_kernels.hpp"
+
+namespace omx {
+
+__global__ void vcopy_k(cplx *y, const cplx *x, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) y[i] = x[i];
+}
+__global__ void vaxpby_k(cplx *y, cplx a, const cplx *x, cplx b, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) y[i] = a * x[i] + b * y[i];
+}
+__global__ void vsub_k(cplx *z, const cplx *x, const cplx *y, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) z[i] = x[i] - y[i];
+}
+// y[i] = x[i] * ov[kind][obj][pol][deg(i)] - sub[i], over the N = 2 nf P unknowns
+__global__ void vmul_ov_k(cplx *y, const cplx *x, const cplx *ov, int kind, int P, int nmax, const cplx *sub,
+                          int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const int nf = nmax * (nmax + 2);
+  int obj = i / (2 * nf), r = i % (2 * nf);
+  int pol = r >= nf, q = r % nf, deg, m;
+  unflat(q, deg, m);
+  cplx v = x[i] * ov[(((size_t)kind * P + obj) * 2 + pol) * (nmax + 1) + deg];
+  if(sub) v = v - sub[i];
+  y[i] = v;
+}
+
+// out[k] = sum_i conj(V[k][i]) w[i]; one partial per block, then reduced.
+constexpr int kRedBlocks = 128;
+__global__ __launch_bounds__(256) void dotc_multi_k(const cplx *V, size_t ldv, int nv, const cplx *w, int n,
+                                                    cplx *part) {
+  __shared__ cplx red[256];
+  for(int k = 0; k < nv; ++k) {
+    cplx s = mk(0.0);
+    for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s = s + conj(V[k * ldv + i]) * w[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for(int st = 128; st > 0; st >>= 1) {
+      if(threadIdx.x < st) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + st];
+      __syncthreads();
+    }
+    if(threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+__global__ void reduce_rows_k(const cplx *part, int nb, int nv, cplx *out) {
+  int k = blockIdx.x;
+  __shared__ cplx red[128];
+  cplx s = mk(0.0);
+  for(int b = threadIdx.x; b < nb; b += 128) s = s + part[(size_t)k * nb + b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for(int st = 64; st > 0; st >>= 1) {
+    if(threadIdx.x < st) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) out[k] = red[0];
+}
+__global__ void axpy_multi_k(const cplx *V, size_t ldv, int nv, const cplx *h, cplx *w, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  cplx s = w[i];
+  for(int k = 0; k < nv; ++k) s = s - V[k * ldv + i] * h[k];
+  w[i] = s;
+}
+__global__ void gemv_multi_k(const cplx *V, size_t ldv, int nv, const cplx *h, cplx *z, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  cplx s = mk(0.0);
+  for(int k = 0; k < nv; ++k) s = s + V[k * ldv + i] * h[k];
+  z[i] = s;
+}
+__global__ __launch_bounds__(256) void nrm2_k(const cplx *x, int n, double *part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += norm2(x[i]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for(int st = 128; st > 0; st >>= 1) {
+    if(threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ void nrm2_reduce_k(const double *part, int nb, double *out) {
+  __shared__ double red[128];
+  double s = 0.0;
+  for(int b = threadIdx.x; b < nb; b += 128) s += part[b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for(int st = 64; st > 0; st >>= 1) {
+    if(threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) *out = red[0];
+}
+
+static inline dim3 g1(int n) { return dim3((n + 255) / 256); }
+
+void vcopy(cplx *y, const cplx *x, int n, hipStream_t st) { hipLaunchKernelGGL(vcopy_k, g1(n), dim3(256), 0, st, y, x, n); }
+void vaxpby(cplx *y, cplx a, const cplx *x, cplx b, int n, hipStream_t st) {
+  hipLaunchKernelGGL(vaxpby_k, g1(n), dim3(256), 0, st, y, a, x, b, n);
+}
+void vsub(cplx *z, const cplx *x, const cplx *y, int n, hipStream_t st) {
+  hipLaunchKernelGGL(vsub_k, g1(n), dim3(256), 0, st, z, x, y, n);
+}
+void vmul_ov(cplx *y, const cplx *x, const cplx *ov, int kind, int P, int nmax, const cplx *sub, int n,
+             hipStream_t st) {
+  hipLaunchKernelGGL(vmul_ov_k, g1(n), dim3(256), 0, st, y, x, ov, kind, P, nmax, sub, n);
+}
+void dotc_multi(const cplx *V, size_t ldv, int nv, const cplx *w, int n, cplx *part, cplx *out, hipStream_t st) {
+  int nb = std::min(kRedBlocks, (n + 255) / 256);
+  hipLaunchKernelGGL(dotc_multi_k, dim3(nb), dim3(256), 0, st, V, ldv, nv, w, n, part);
+  hipLaunchKernelGGL(reduce_rows_k, dim3(nv), dim3(128), 0, st, part, nb, nv, out);
+}
+void axpy_multi(const cplx *V, size_t ldv, int nv, const cplx *h, cplx *w, int n, hipStream_t st) {
+  hipLaunchKernelGGL(axpy_multi_k, g1(n), dim3(256), 0, st, V, ldv, nv, h, w, n);
+}
+void gemv_multi(const cplx *V, size_t ldv, int nv, const cplx *h, cplx *z, int n, hipStream_t st) {
+  hipLaunchKernelGGL(gemv_multi_k, g1(n), dim3(256), 0, st, V, ldv, nv, h, z, n);
+}
+void nrm2sq(const cplx *x, int n, double *part, double *out, hipStream_t st) {
+  int nb = std::min(kRedBlocks, (n + 255) / 256);
+  hipLaunchKernelGGL(nrm2_k, dim3(nb), dim3(256), 0, st, x, n, part);
+  hipLaunchKernelGGL(nrm2_reduce_k, dim3(1), dim3(128), 0, st, part, nb, out);
+}
+
+}  // namespace omx

@@ -1,0 +1,125 @@
+// omx_kernels.hpp -- launch interfaces shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "omx_math.hpp"
+
+namespace omx {
+
+// per-object vector kinds, layout ov[kind][obj][pol][n], pol 0 = TE, 1 = TM
+enum {
+  OV_T = 0,      // getTLocal
+  OV_TSH,        // getTLocalSH
+  OV_IAUX,       // getIaux
+  OV_IAUXSH1,    // getIauxSH1
+  OV_IAUXSH2,    // getIauxSH2
+  OV_TSH1O,      // getTLocalSH1_outer
+  OV_TSH2O,      // getTLocalSH2_outer
+  OV_CABS,       // getCabsAux (real, in .x)
+  OV_SHRAD,      // [0][n] = j_n(k_j R), [1][n] = k_j j_n'(k_j R) + j_n / R
+  OV_COUNT
+};
+
+constexpr int kCoefN = 2 * kMaxNmax + 2;
+constexpr int kCoefStride = (kCoefN + 1) * (kCoefN + 1);
+
+struct ObjArgs {
+  int P, nmax;
+  double omega;
+  cplx bg_eps, bg_mu;                // absolute
+  const double *xyz, *radius;
+  const cplx *eps, *mu, *eps_sh;     // absolute
+  double *sph;                       // out: 3P spherical coordinates
+  cplx *ov;                          // out: per-object vectors
+};
+
+struct IncArgs {
+  int nmax;
+  double theta, phi;
+  cplx etheta, ephi;
+  cplx *inc;                         // out: [a_inc (nf); b_inc (nf)]
+};
+
+struct PairArgs {
+  int P, nmax, tkind;                // tkind: OV_T for S, OV_TSH for V
+  cplx k;                            // background wavenumber (k or 2k)
+  const double *sph;
+  const cplx *ov;
+  const double *coef;
+  size_t lda;
+  cplx *S64;                         // optional FP64 output
+  float2 *S32;                       // optional complex-FP32 output
+};
+
+struct OriginArgs {
+  int P, nmax;
+  cplx k;
+  const double *sph;
+  const cplx *ov;
+  const double *coef;
+  const cplx *inc;
+  cplx *incloc;                      // out: N
+  cplx *Q;                           // out: N
+  double *w;                         // out: [2][P][nf]
+};
+
+struct CrossArgs {
+  int P, nmax;
+  const cplx *x, *xsh, *incloc;
+  const double *w;
+  const cplx *ov;
+  double *out;                       // [P][4]
+};
+
+void run_objects(const ObjArgs &a, hipStream_t st);
+void run_incident(const IncArgs &a, hipStream_t st);
+void run_pairs(const PairArgs &a, hipStream_t st);
+void run_origin(const OriginArgs &a, hipStream_t st);
+void run_cross(const CrossArgs &a, hipStream_t st);
+void run_pad64(cplx *A, int N, int Np, size_t lda, hipStream_t st);
+void run_pad32(float2 *A, int N, int Np, size_t lda, hipStream_t st);
+void fill_coef(double *host);
+
+// ---------------------------------------------------------------- LU
+constexpr int kNB = 64;        // LU block size (panel width, padding granule)
+constexpr int kPanelRows = 128; // rows of the panel held by one workgroup
+
+struct LuWork {
+  // device scratch of one factorization
+  int *piv;             // Np
+  int *perm;            // Np, final row permutation
+  double *cand_val;     // [2][Gmax]
+  int *cand_idx;        // [2][Gmax]
+  void *cand_row;       // [2][Gmax][NB] complex
+  void *rowj;           // [2][NB] complex
+  unsigned *bar;        // grid-barrier counter
+  int *status;          // [0]: singular pivot column+1, [1]: wait timeout
+  int *flags;           // Np / NB triangular-solve flags
+  int gmax;
+  int epoch;            // last flag value used by a triangular solve
+};
+
+// Factor the Np x Np complex matrix in place (P A = L U); T = double2 or float2.
+template <typename T>
+int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st);
+// Solve with the factors: x = A^{-1} b, vectors always complex FP64.
+template <typename T>
+void lu_solve(const T *A, int Np, size_t lda, LuWork &w, const cplx *b, cplx *x, hipStream_t st);
+
+// ---------------------------------------------------------------- iterative
+void gemv64(const cplx *A, size_t lda, int N, const cplx *x, cplx *y, cplx *part, hipStream_t st);
+constexpr int kGemvSplitMax = 32;
+
+// level-1 helpers (omx_iter.hip); reductions return through small device arrays
+void vcopy(cplx *y, const cplx *x, int n, hipStream_t st);
+void vaxpby(cplx *y, cplx a, const cplx *x, cplx b, int n, hipStream_t st);   // y = a x + b y
+void vsub(cplx *z, const cplx *x, const cplx *y, int n, hipStream_t st);      // z = x - y
+void vmul_ov(cplx *y, const cplx *x, const cplx *ov, int kind, int P, int nmax, const cplx *sub, int n,
+             hipStream_t st);                                                  // y = x (.) ov[kind] - sub
+void dotc_multi(const cplx *V, size_t ldv, int nv, const cplx *w, int n, cplx *part, cplx *out,
+                hipStream_t st);                                               // out[k] = V_k^H w
+void axpy_multi(const cplx *V, size_t ldv, int nv, const cplx *h, cplx *w, int n, hipStream_t st); // w -= V h
+void gemv_multi(const cplx *V, size_t ldv, int nv, const cplx *h, cplx *z, int n, hipStream_t st); // z = V h
+void nrm2sq(const cplx *x, int n, double *part, double *out, hipStream_t st);
+
+}  // namespace omx

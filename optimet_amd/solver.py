@@ -1,0 +1,201 @@
+"""Python mirror of the solver boundary, over the C ABI of liboptimet_mi355x.so.
+
+``Solver`` follows optimet::solver::AbstractSolver (srcAna/Solver.h:62-144):
+construct it from a cluster (Geometry + Excitation), call ``update()``
+(assembly of Q, S and, for the second harmonic, V) and then ``solve()``, which
+returns the scattered and internal coefficients at ω and 2ω in the reference's
+layout.  The HIP library is the only compute path: if it is missing or no GPU
+is present the calls raise, they never fall back to a CPU implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboptimet_mi355x.so")
+
+LU64, LU32_IR, GMRES64 = 0, 1, 2
+KINDS = {"lu64": LU64, "lu32ir": LU32_IR, "gmres64": GMRES64}
+
+ERRORS = {-1: "argument", -2: "device", -3: "memory", -4: "singular", -5: "state", -6: "no convergence",
+          -7: "overlap"}
+
+EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "omx_set_excitation",
+           "omx_set_solver", "omx_set_cg_tables", "omx_build_cg_tables", "omx_update", "omx_solve",
+           "omx_cross_sections", "omx_scattering_size", "omx_get_stats", "omx_read_matrix", "omx_read_vector",
+           "omx_dense_solve"]
+
+
+class OmxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{ERRORS.get(code, code)}] {msg}")
+        self.code = code
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_int), ("Np", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("iterations_sh", ctypes.c_int), ("residual", ctypes.c_double), ("residual_sh", ctypes.c_double),
+                ("t_update_ms", ctypes.c_double), ("t_factor_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double),
+                ("t_sh_source_ms", ctypes.c_double), ("t_assemble_ms", ctypes.c_double),
+                ("device_bytes", ctypes.c_size_t)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m optimet_amd.build`")
+    L = ctypes.CDLL(path)
+    P, D, I, V = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_void_p
+    L.omx_last_error.restype = ctypes.c_char_p
+    L.omx_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), I]
+    L.omx_destroy.argtypes = [P]
+    L.omx_destroy.restype = None
+    L.omx_set_geometry.argtypes = [P, I, I, D, D, D, D, D, D, D, D, D, D]
+    L.omx_set_excitation.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double, D, D, I]
+    L.omx_set_solver.argtypes = [P, I, ctypes.c_double, I, I]
+    L.omx_set_cg_tables.argtypes = [P, I, ctypes.POINTER(D)]
+    L.omx_build_cg_tables.argtypes = [P, I]
+    L.omx_update.argtypes = [P]
+    L.omx_solve.argtypes = [P, D, D, D, D]
+    L.omx_cross_sections.argtypes = [P, D]
+    L.omx_scattering_size.argtypes = [P]
+    L.omx_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
+    L.omx_read_matrix.argtypes = [P, I, D]
+    L.omx_read_vector.argtypes = [P, I, D]
+    L.omx_dense_solve.argtypes = [P, I, D, D, D, I]
+    _lib = L
+    return L
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _c(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+
+
+def _check(rc):
+    if rc != 0:
+        raise OmxError(rc, load_library().omx_last_error().decode())
+
+
+class Context:
+    """One device context (one HIP stream, one set of device buffers)."""
+
+    def __init__(self, device=0):
+        self.L = load_library()
+        h = ctypes.c_void_p()
+        _check(self.L.omx_create(ctypes.byref(h), device))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.omx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self):
+        s = Stats()
+        _check(self.L.omx_get_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def dense_solve(self, A, b, kind="lu64"):
+        A = np.asfortranarray(A, dtype=np.complex128)
+        N = A.shape[0]
+        flat = A.reshape(-1, order="F").copy()
+        bb = _c(b)
+        x = np.zeros(N, dtype=np.complex128)
+        _check(self.L.omx_dense_solve(self.h, N, _d(flat.view(np.float64)), _d(bb.view(np.float64)),
+                                      _d(x.view(np.float64)), KINDS[kind]))
+        return x
+
+
+class Solver:
+    """MI355X backend behind the AbstractSolver interface (Solver.h:62-144).
+
+    kind: "lu64" (FP64 LU), "lu32ir" (complex-FP32 MFMA LU + FP64 GMRES-IR) or
+    "gmres64" (GMRES on the FP64 matrix).
+    """
+
+    def __init__(self, cluster, kind="lu64", tol=1e-12, maxit=300, restart=40, context=None):
+        self.ctx = context or Context()
+        self.kind = kind
+        _check(self.ctx.L.omx_set_solver(self.ctx.h, KINDS[kind], tol, maxit, restart))
+        self.set_cluster(cluster)
+
+    def set_cluster(self, c):
+        """AbstractSolver::update(geometry, incWave) inputs (Solver.h:94-99)."""
+        self.cluster = c
+        L, h = self.ctx.L, self.ctx.h
+        self._keep = [np.ascontiguousarray(c.xyz.reshape(-1)), np.ascontiguousarray(c.radius),
+                      _c(c.eps_r), _c(c.mu_r), _c(c.eps_r_sh), _c(c.ksippp), _c(c.ksiparppar), _c(c.gamma),
+                      _c([c.bg_eps_r]), _c([c.bg_mu_r])]
+        k = self._keep
+        args = [_d(k[0]), _d(k[1])] + [_d(a.view(np.float64)) for a in k[2:]]
+        _check(L.omx_set_geometry(h, c.nobj, c.nmax, *args))
+        et, ep = _c([c.etheta]), _c([c.ephi])
+        _check(L.omx_set_excitation(h, c.wavelength, c.theta_k, c.phi_k, _d(et.view(np.float64)),
+                                    _d(ep.view(np.float64)), 1 if c.sh else 0))
+
+    def set_cg_tables(self, tables):
+        """The CGcoeff argument of solve() (9 tables [k][p][q])."""
+        t = [np.ascontiguousarray(tables[i], dtype=np.float64).reshape(-1) for i in range(9)]
+        arr = (ctypes.POINTER(ctypes.c_double) * 9)(*[_d(x) for x in t])
+        _check(self.ctx.L.omx_set_cg_tables(self.ctx.h, self.cluster.nmax, arr))
+
+    def update(self):
+        _check(self.ctx.L.omx_update(self.ctx.h))
+
+    def solve(self):
+        """Returns dict x_sca, x_int (and x_sca_sh, x_int_sh with SH)."""
+        N = self.cluster.N
+        xs, xi = np.zeros(N, np.complex128), np.zeros(N, np.complex128)
+        xss, xis = (np.zeros(N, np.complex128), np.zeros(N, np.complex128)) if self.cluster.sh else (None, None)
+        _check(self.ctx.L.omx_solve(self.ctx.h, _d(xs.view(np.float64)), _d(xi.view(np.float64)),
+                                    None if xss is None else _d(xss.view(np.float64)),
+                                    None if xis is None else _d(xis.view(np.float64))))
+        out = {"x_sca": xs, "x_int": xi}
+        if self.cluster.sh:
+            out.update({"x_sca_sh": xss, "x_int_sh": xis})
+        return out
+
+    def cross_sections(self):
+        cs = np.zeros(4)
+        _check(self.ctx.L.omx_cross_sections(self.ctx.h, _d(cs)))
+        return {"cext": cs[0], "csca": cs[1], "cabs": cs[2], "csca_sh": cs[3]}
+
+    def matrix(self, sh=False):
+        N = self.cluster.N
+        out = np.zeros(N * N, np.complex128)
+        _check(self.ctx.L.omx_read_matrix(self.ctx.h, 1 if sh else 0, _d(out.view(np.float64))))
+        return out.reshape(N, N).T
+
+    def vector(self, which):
+        """which: "Q", "K", "K1", "inc"."""
+        N = self.cluster.N
+        out = np.zeros(N, np.complex128)
+        idx = {"Q": 0, "K": 1, "K1": 2, "inc": 3}[which]
+        _check(self.ctx.L.omx_read_vector(self.ctx.h, idx, _d(out.view(np.float64))))
+        return out
+
+    def stats(self):
+        return self.ctx.stats()
+
+    def scattering_size(self):
+        return self.ctx.L.omx_scattering_size(self.ctx.h)
