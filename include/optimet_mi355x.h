@@ -67,6 +67,12 @@ typedef struct {
   double t_sh_source_ms; /* last omx_solve: second-harmonic source vector */
   double t_assemble_ms;  /* last omx_update: pair-coefficient kernels only */
   size_t device_bytes;   /* device memory held by the context */
+  /* with omx_set_profile(ctx, 1): per-kernel time on the context's stream */
+  double t_lu_gemm_ms;   /* trailing-update GEMM launches of the last factorisations */
+  double lu_gemm_flop;   /* their algorithmic flop, 8 m n k per launch */
+  int lu_gemm_launches;
+  double t_lu_panel_ms;  /* panel factorisation launches */
+  double t_factor_total_ms; /* whole factorisations (panel, swaps, TRSM, GEMM) */
 } omx_stats;
 
 const char *omx_last_error(void);
@@ -122,6 +128,10 @@ int omx_cross_sections(omx_context *ctx, double *out4);
 int omx_scattering_size(const omx_context *ctx);
 
 int omx_get_stats(const omx_context *ctx, omx_stats *out);
+
+/* Per-kernel timing of the factorisations with HIP events (off by default:
+ * it adds event records between launches). */
+int omx_set_profile(omx_context *ctx, int on);
 
 /* Parity access: copies the assembled N x N matrix (which 0: S, 1: V; FP64,
  * column-major) or a vector (which 0: Q, 1: K, 2: K1ana, 3: local incident

@@ -17,7 +17,9 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed-FP32 instructions stay disabled: the compiler would otherwise form
 # v_pk_*_f32 from scalar float code.
 COMMON = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unused-result",
-          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          # full unrolling keeps the register-resident panel rows out of scratch
+          "-mllvm", "-pragma-unroll-threshold=1000000"]
 
 
 def _compile(src):

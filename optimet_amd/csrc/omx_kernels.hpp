@@ -86,22 +86,32 @@ constexpr int kPanelRows = 128; // rows of the panel held by one workgroup
 
 struct LuWork {
   // device scratch of one factorization
-  int *piv;             // Np
-  int *perm;            // Np, final row permutation
-  double *cand_val;     // [2][Gmax]
-  int *cand_idx;        // [2][Gmax]
-  void *cand_row;       // [2][Gmax][NB] complex
-  void *rowj;           // [2][NB] complex
-  unsigned *bar;        // grid-barrier counter
+  int *piv;             // Np: LAPACK-style swap sequence
+  int *perm;            // Np: final row permutation (row i of P A is row perm[i] of A)
+  int *cand[2];         // tournament candidates, ping-pong, [Gmax][NB]
+  int *cnt[2];          // candidates per set, [Gmax]
+  int *mdst, *msrc;     // gather map of one panel, 2 NB entries
+  int *mcnt;
   int *status;          // [0]: singular pivot column+1, [1]: wait timeout
   int *flags;           // Np / NB triangular-solve flags
+  void *wlu;            // 3 x NB x NB complex: selected-block factors, L11^{-1}, U11^{-1}
+  hipStream_t aux;      // look-ahead stream of the panel tournaments
+  hipEvent_t ev_next, ev_piv;
   int gmax;
   int epoch;            // last flag value used by a triangular solve
 };
 
+// Optional per-launch timing of the factorisation (events on the same stream).
+struct LuProf {
+  hipEvent_t *ev = nullptr;  // 4 events per panel: panel start/end, gemm start/end
+  int cap = 0;               // panels the pool covers
+  int used = 0;              // panels recorded by the last factorisation
+  double gemm_flop = 0.0;    // algorithmic flop of the recorded trailing updates
+};
+
 // Factor the Np x Np complex matrix in place (P A = L U); T = double2 or float2.
 template <typename T>
-int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st);
+int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof = nullptr);
 // Solve with the factors: x = A^{-1} b, vectors always complex FP64.
 template <typename T>
 void lu_solve(const T *A, int Np, size_t lda, LuWork &w, const cplx *b, cplx *x, hipStream_t st);

@@ -24,7 +24,7 @@ ERRORS = {-1: "argument", -2: "device", -3: "memory", -4: "singular", -5: "state
 EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "omx_set_excitation",
            "omx_set_solver", "omx_set_cg_tables", "omx_build_cg_tables", "omx_update", "omx_solve",
            "omx_cross_sections", "omx_scattering_size", "omx_get_stats", "omx_read_matrix", "omx_read_vector",
-           "omx_dense_solve"]
+           "omx_dense_solve", "omx_set_profile"]
 
 
 class OmxError(RuntimeError):
@@ -38,7 +38,9 @@ class Stats(ctypes.Structure):
                 ("iterations_sh", ctypes.c_int), ("residual", ctypes.c_double), ("residual_sh", ctypes.c_double),
                 ("t_update_ms", ctypes.c_double), ("t_factor_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double),
                 ("t_sh_source_ms", ctypes.c_double), ("t_assemble_ms", ctypes.c_double),
-                ("device_bytes", ctypes.c_size_t)]
+                ("device_bytes", ctypes.c_size_t), ("t_lu_gemm_ms", ctypes.c_double),
+                ("lu_gemm_flop", ctypes.c_double), ("lu_gemm_launches", ctypes.c_int),
+                ("t_lu_panel_ms", ctypes.c_double), ("t_factor_total_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -73,6 +75,7 @@ def load_library(path=LIB_PATH):
     L.omx_read_matrix.argtypes = [P, I, D]
     L.omx_read_vector.argtypes = [P, I, D]
     L.omx_dense_solve.argtypes = [P, I, D, D, D, I]
+    L.omx_set_profile.argtypes = [P, I]
     _lib = L
     return L
 
@@ -110,6 +113,9 @@ class Context:
         except Exception:
             pass
 
+    def set_profile(self, on=True):
+        _check(self.L.omx_set_profile(self.h, 1 if on else 0))
+
     def stats(self):
         s = Stats()
         _check(self.L.omx_get_stats(self.h, ctypes.byref(s)))
@@ -133,7 +139,7 @@ class Solver:
     "gmres64" (GMRES on the FP64 matrix).
     """
 
-    def __init__(self, cluster, kind="lu64", tol=1e-12, maxit=300, restart=40, context=None):
+    def __init__(self, cluster, kind="lu64", tol=1e-16, maxit=300, restart=40, context=None):
         self.ctx = context or Context()
         self.kind = kind
         _check(self.ctx.L.omx_set_solver(self.ctx.h, KINDS[kind], tol, maxit, restart))

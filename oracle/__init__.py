@@ -81,6 +81,7 @@ def _declare(L):
     L.ora_source_sh.argtypes = [_P, _D, _D, _D, _D]
     L.ora_solve_all.argtypes = [_P, _I, _D, _D, _D, _D, _D]
     L.ora_cross_sections.argtypes = [_P, _D, _D, _D]
+    L.ora_time_sample.argtypes = [_P, _I, _I, _I, _D]
 
 
 def _ptr(a):
@@ -280,3 +281,11 @@ def cross_sections(prob, x_sca, x_sca_sh=None):
         _check(lib().ora_cross_sections(p.ref(), _ptr(xs.view(np.float64)),
                                         _ptr(xh.view(np.float64)), _ptr(cs)))
     return cs
+
+
+def time_sample(prob, npairs, nsolve, method=1):
+    """(s per pair block, s per dense solve of order nsolve, s per object SH source)."""
+    out = np.zeros(3)
+    p = _Pinned(prob)
+    _check(lib().ora_time_sample(p.ref(), npairs, nsolve, method, _ptr(out)))
+    return tuple(out)

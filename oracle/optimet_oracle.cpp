@@ -20,6 +20,7 @@
 #include "optimet_oracle.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -1196,6 +1197,57 @@ int ora_solve_all(const ora_problem *in, int method, double *x_sca, double *x_in
       std::memcpy(x_int_sh, XIS.data(), N * sizeof(cd));
     }
     if(xs) cross_sections(pb, X.data(), pb.sh ? XS.data() : nullptr, xs);
+  });
+}
+
+// Timing sample for the CPU baseline: out[0] seconds per ordered pair
+// (irregular Coupling + the -T_i [[A^T,B^T],[B^T,A^T]] block, as
+// PreconditionedMatrix.cpp:350-400), out[1] seconds of a dense solve of order
+// nsolve with `method` (QR as PreconditionedMatrixSolver.h:58), out[2] seconds
+// per object of the second-harmonic source (Geometry.cpp:250-310) with
+// tables of the problem's size.
+int ora_time_sample(const ora_problem *in, int npairs, int nsolve, int method, double *out) {
+  return guard([&] {
+    Problem pb = make_problem(in);
+    int nf = pb.nf;
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    std::vector<cd> blk((size_t)4 * nf * nf);
+    auto T = object_vector(pb, pb.obj[0], 0, pb.bg_eps, pb.bg_mu);
+    auto t0 = now();
+    int done = 0;
+    for(int i = 0; i < pb.P && done < npairs; ++i)
+      for(int j = 0; j < pb.P && done < npairs; ++j) {
+        if(i == j) continue;
+        std::vector<cd> A, B;
+        coupling(sph_minus(pb.obj[i].vR, pb.obj[j].vR), pb.waveK, pb.nmax, true, A, B);
+        for(int c = 0; c < 2 * nf; ++c)
+          for(int r = 0; r < 2 * nf; ++r) {
+            int rr = r % nf, cc = c % nf;
+            bool diag = (r < nf) == (c < nf);
+            blk[(size_t)c * 2 * nf + r] = -T[r] * (diag ? A[(size_t)cc * nf + rr] : B[(size_t)cc * nf + rr]);
+          }
+        ++done;
+      }
+    out[0] = std::chrono::duration<double>(now() - t0).count() / std::max(done, 1);
+    std::vector<cd> S((size_t)nsolve * nsolve), b(nsolve), x;
+    unsigned long long st = 88172645463325252ull;
+    auto rnd = [&st] {
+      st ^= st << 13;
+      st ^= st >> 7;
+      st ^= st << 17;
+      return (double)(st >> 11) / 9007199254740992.0 - 0.5;
+    };
+    for(auto &v : S) v = cd(rnd(), rnd());
+    for(int i = 0; i < nsolve; ++i) S[(size_t)i * nsolve + i] += (double)nsolve;
+    for(auto &v : b) v = cd(rnd(), rnd());
+    t0 = now();
+    dense_solve(nsolve, S, b, x, method);
+    out[1] = std::chrono::duration<double>(now() - t0).count();
+    std::vector<double> tab((size_t)9 * nf * nf * nf, 0.0);
+    std::vector<cd> u((size_t)2 * nf * pb.P, cd(1.0, 0.5)), ana(4 * nf);
+    t0 = now();
+    inc_local_sh(pb, tab.data(), u, 0, ana.data());
+    out[2] = std::chrono::duration<double>(now() - t0).count();
   });
 }
 

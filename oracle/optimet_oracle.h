@@ -105,6 +105,11 @@ int ora_solve_all(const ora_problem *pb, int method, double *x_sca, double *x_in
 int ora_cross_sections(const ora_problem *pb, const double *x_sca, const double *x_sca_sh,
                        double *xs);
 
+/* CPU-baseline timing sample: out[0] s per pair coupling + block, out[1] s
+ * of one dense solve of order nsolve (method as ora_dense_solve), out[2] s of
+ * one object's second-harmonic source. */
+int ora_time_sample(const ora_problem *pb, int npairs, int nsolve, int method, double *out);
+
 #ifdef __cplusplus
 }
 #endif

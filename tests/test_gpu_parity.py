@@ -26,6 +26,16 @@ def small_drude(P=5, nmax=4, sh=True, seed=11):
     return drude_cluster(P, nmax, 700e-9, 40, 80, 600, 10, seed=seed, sh=sh)
 
 
+def check_internal(ora, c, out, ref, tol):
+    """x_int = x_sca (.) Iaux (Solver.cpp:57-77).  Iaux grows like (2n+1)!!/x^n
+    with the harmonic order, so it weights the smallest scattered coefficients
+    most: the unpreconditioning step itself is pinned exactly, and x_int is held
+    to the coefficient bar times the largest relative weight of Iaux (<= 100)."""
+    iaux = np.concatenate([ora.object_vector(c, j, ora.OBJ_IAUX) for j in range(c.nobj)])
+    assert rel(out["x_int"], out["x_sca"] * iaux) < 1e-12
+    assert rel(out["x_int"], ref["x_int"]) < 100 * tol
+
+
 def make(ctx, c, kind="lu64"):
     from optimet_amd import solver
     s = solver.Solver(c, kind=kind, context=ctx)
@@ -79,7 +89,7 @@ def test_fundamental_solve(ctx, ora, name, kind, tol):
     s = make(ctx, c, kind)
     out = s.solve()
     assert rel(out["x_sca"], ref["x_sca"]) < tol
-    assert rel(out["x_int"], ref["x_int"]) < tol
+    check_internal(ora, c, out, ref, tol)
     cs = s.cross_sections()
     for k in ("cext", "csca", "cabs"):
         assert abs(cs[k] - ref[k]) <= tol * abs(ref[k]), k
