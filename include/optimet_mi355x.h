@@ -48,9 +48,15 @@ extern "C" {
 #define OMX_ERR_OVERLAP -7   /* overlapping spheres (Geometry.cpp:39-51) */
 
 /* Solver kinds: the backend selected from the XML input. */
-#define OMX_SOLVER_LU64 0     /* complex FP64 LU with partial pivoting */
+#define OMX_SOLVER_LU64 0     /* complex FP64 LU (+ FP64 refinement after block pivoting) */
 #define OMX_SOLVER_LU32_IR 1  /* complex FP32 LU on MFMA + FP64 GMRES-IR */
 #define OMX_SOLVER_GMRES64 2  /* restarted GMRES, FP64 matrix streamed from HBM */
+
+/* Pivot search of the LU factorisations (omx_set_pivoting). */
+#define OMX_PIVOT_BLOCK 0     /* among the rows of each 64 x 64 diagonal block, then
+                                 FP64 GMRES-IR; falls back to OMX_PIVOT_PARTIAL on a
+                                 zero pivot or a refinement that does not converge */
+#define OMX_PIVOT_PARTIAL 1   /* partial pivoting over the whole column (tournament) */
 
 typedef struct omx_context omx_context;
 
@@ -73,6 +79,7 @@ typedef struct {
   int lu_gemm_launches;
   double t_lu_panel_ms;  /* panel factorisation launches */
   double t_factor_total_ms; /* whole factorisations (panel, swaps, TRSM, GEMM) */
+  int pivot_fallbacks;   /* refactorisations with partial pivoting since creation */
 } omx_stats;
 
 const char *omx_last_error(void);
@@ -103,6 +110,13 @@ int omx_set_excitation(omx_context *ctx, double wavelength, double theta, double
  * parts (GMRES-IR refinement, GMRES), maxit their iteration cap, restart the
  * GMRES restart length. */
 int omx_set_solver(omx_context *ctx, int kind, double tol, int maxit, int restart);
+
+/* Pivot search of the LU solvers (default OMX_PIVOT_BLOCK).  The reference
+ * solves with Eigen's ColPivHouseholderQR (PreconditionedMatrixSolver.h:58,
+ * 75) or ScaLAPACK pzgesv (scalapack/LinearSystemSolver.hpp:84-91); both
+ * modes reach the same FP64 residual.  OMX_PIVOT_PARTIAL with
+ * OMX_SOLVER_LU64 is the plain partial-pivoting FP64 LU solve. */
+int omx_set_pivoting(omx_context *ctx, int mode);
 
 /* The nine Clebsch-Gordan / Wigner tables of the second-harmonic source
  * (CGcoeff, Simulation.cpp:613-616), each nf*nf*nf doubles in the order

@@ -81,11 +81,14 @@ struct omx_context {
   int sh = 0;
   // solver
   int kind = OMX_SOLVER_LU64;
+  int pivot = OMX_PIVOT_BLOCK;
+  int fallbacks = 0;
   double tol = 1e-16;
   int maxit = 300, restart = 40;
   // state
   bool updated = false, solved = false;
   bool factored = false, factored_sh = false;
+  bool dirty32 = false, dirty32_sh = false;  // the FP32 matrix holds factors, not S (V)
   int cg_nmax = 0;
   // device buffers
   DBuf d_xyz, d_radius, d_eps, d_mu, d_eps_sh, d_kppp, d_kpar, d_gam, d_sph, d_ov, d_inc, d_incloc, d_Q, d_w,
@@ -101,6 +104,7 @@ struct omx_context {
   double prof_gemm_ms = 0, prof_panel_ms = 0, prof_flop = 0, prof_total_ms = 0;
   int prof_launches = 0;
   DBuf d_piv2, d_perm2, d_flags2;
+  DBuf d_dinv, d_dinv2, d_F64, d_F64sh;  // diagonal-block inverses; FP64 factors (the FP64 matrices stay intact)
   omx_stats stats{};
   double omega() const { return omx::kC0 * (2.0 * omx::kPi / lambda); }
   cplx waveK() const {
@@ -130,10 +134,11 @@ cplx C(const double *p, int i) { return omx::mk(p[2 * i], p[2 * i + 1]); }
 
 bool use32(int kind) { return kind == OMX_SOLVER_LU32_IR; }
 
-void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags, bool second) {
+void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags, DBuf &dinv, bool second) {
   const int Np = c->Np;
   const int gmax = (Np + omx::kPanelRows - 1) / omx::kPanelRows + 1;
   piv.ensure(sizeof(int) * Np, &c->total);
+  dinv.ensure(sizeof(cplx) * 2 * omx::kNB * (size_t)Np, &c->total);
   perm.ensure(sizeof(int) * Np, &c->total);
   flags.ensure(sizeof(int) * (Np / omx::kNB + 1), &c->total);
   if(!second) {
@@ -158,6 +163,8 @@ void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags
   w.mcnt = c->d_map.as<int>() + 4 * omx::kNB;
   w.status = c->d_status.as<int>();
   w.wlu = c->d_wlu.p;
+  w.dinv = dinv.p;
+  w.pivot = c->pivot;
   w.aux = c->aux;
   w.ev_next = c->lu_ev[0];
   w.ev_piv = c->lu_ev[1];
@@ -213,9 +220,12 @@ void alloc_matrices(omx_context *c) {
     c->d_V64.release(T);
     c->d_V32.release(T);
   }
-  if(c->kind != OMX_SOLVER_LU64) c->d_basis.ensure(sizeof(cplx) * (size_t)c->Np * (c->restart + 1), T);
-  alloc_lu(c, c->lu, c->d_piv, c->d_perm, c->d_flags, false);
-  if(c->sh) alloc_lu(c, c->lu_sh, c->d_piv2, c->d_perm2, c->d_flags2, true);
+  const bool f64 = c->kind == OMX_SOLVER_LU64;
+  if(f64) c->d_F64.ensure(sizeof(cplx) * n2, T); else c->d_F64.release(T);
+  if(f64 && c->sh) c->d_F64sh.ensure(sizeof(cplx) * n2, T); else c->d_F64sh.release(T);
+  c->d_basis.ensure(sizeof(cplx) * (size_t)c->Np * (c->restart + 1), T);
+  alloc_lu(c, c->lu, c->d_piv, c->d_perm, c->d_flags, c->d_dinv, false);
+  if(c->sh) alloc_lu(c, c->lu_sh, c->d_piv2, c->d_perm2, c->d_flags2, c->d_dinv2, true);
 }
 
 float elapsed(omx_context *c, int a, int b) {
@@ -387,48 +397,78 @@ void prof_end(omx_context *c) {
 void solve_system(omx_context *c, bool shf, const cplx *rhs, cplx *x, int *iters, double *res) {
   omx::LuWork &lu = shf ? c->lu_sh : c->lu;
   cplx *A64 = (shf ? c->d_V64 : c->d_S64).as<cplx>();
-  float2 *A32 = (shf ? c->d_V32 : c->d_S32).as<float2>();
   bool &fact = shf ? c->factored_sh : c->factored;
   const int Np = c->Np;
-  const size_t lda = (size_t)Np;
-  HIPCHK(hipMemsetAsync(c->d_status.p, 0, 16, c->st));
-  if(c->kind == OMX_SOLVER_LU64) {
-    if(!fact) {
-      prof_begin(c);
-      int rc = omx::lu_factor<double2>(reinterpret_cast<double2 *>(A64), Np, lda, lu, c->st,
-                                       c->profile ? &c->prof : nullptr);
-      if(rc) throw OmxError(OMX_ERR_DEVICE, "lu_factor launch failed (" + std::to_string(rc) + ")");
-      prof_end(c);
-      fact = true;
-    }
-    omx::lu_solve<double2>(reinterpret_cast<const double2 *>(A64), Np, lda, lu, rhs, x, c->st);
+  const size_t lda = (size_t)Np, n2 = lda * Np;
+  Gmres gm{c, A64, nullptr, Np};
+  if(c->kind == OMX_SOLVER_GMRES64) {
+    HIPCHK(hipMemsetAsync(c->d_status.p, 0, 16, c->st));
+    HIPCHK(hipMemsetAsync(x, 0, sizeof(cplx) * Np, c->st));
+    *iters = gm.solve(rhs, x, c->tol, c->maxit, c->restart, res);
     check_status(c);
-    *iters = 0;
-    *res = -1.0;
+    if(!(*res <= 1e3 * c->tol))
+      throw OmxError(OMX_ERR_NOCONV, "GMRES stopped at relative residual " + std::to_string(*res));
     return;
   }
-  Gmres gm{c, A64, nullptr, Np};
-  if(c->kind == OMX_SOLVER_LU32_IR) {
-    if(!fact) {
-      prof_begin(c);
-      int rc = omx::lu_factor<float2>(A32, Np, lda, lu, c->st, c->profile ? &c->prof : nullptr);
-      if(rc) throw OmxError(OMX_ERR_DEVICE, "lu_factor launch failed (" + std::to_string(rc) + ")");
-      prof_end(c);
-      fact = true;
+  const bool f64 = c->kind == OMX_SOLVER_LU64;
+  cplx *F64 = (shf ? c->d_F64sh : c->d_F64).as<cplx>();
+  float2 *A32 = (shf ? c->d_V32 : c->d_S32).as<float2>();
+  // Block pivoting first (when selected); a zero pivot or a refinement that
+  // does not converge falls back to tournament (partial) pivoting.
+  bool &dirty = shf ? c->dirty32_sh : c->dirty32;
+  const int first = fact ? lu.pivot : c->pivot;
+  for(int pv = first; pv <= OMX_PIVOT_PARTIAL; ++pv) {
+    const bool last = pv == OMX_PIVOT_PARTIAL;
+    try {
+      HIPCHK(hipMemsetAsync(c->d_status.p, 0, 16, c->st));
+      if(!fact || lu.pivot != pv) {
+        lu.pivot = pv;
+        prof_begin(c);
+        int rc;
+        if(f64) {
+          HIPCHK(hipMemcpyAsync(F64, A64, sizeof(cplx) * n2, hipMemcpyDeviceToDevice, c->st));
+          rc = omx::lu_factor<double2>(reinterpret_cast<double2 *>(F64), Np, lda, lu, c->st,
+                                       c->profile ? &c->prof : nullptr);
+        } else {
+          if(dirty) omx::demote(A32, A64, n2, c->st);  // an earlier factorisation overwrote it
+          dirty = true;
+          rc = omx::lu_factor<float2>(A32, Np, lda, lu, c->st, c->profile ? &c->prof : nullptr);
+        }
+        if(rc) throw OmxError(OMX_ERR_DEVICE, "lu_factor launch failed (" + std::to_string(rc) + ")");
+        prof_end(c);
+        fact = true;
+        check_status(c);
+      }
+      if(f64) {
+        gm.prec = [c, F64, &lu, Np, lda](const cplx *in, cplx *out) {
+          omx::lu_solve<double2>(reinterpret_cast<const double2 *>(F64), Np, lda, lu, in, out, c->st);
+        };
+        omx::lu_solve<double2>(reinterpret_cast<const double2 *>(F64), Np, lda, lu, rhs, x, c->st);
+        if(pv == OMX_PIVOT_PARTIAL) {  // plain FP64 LU with partial pivoting
+          check_status(c);
+          *iters = 0;
+          *res = -1.0;
+          return;
+        }
+      } else {
+        gm.prec = [c, A32, &lu, Np, lda](const cplx *in, cplx *out) {
+          omx::lu_solve<float2>(A32, Np, lda, lu, in, out, c->st);
+        };
+        omx::lu_solve<float2>(A32, Np, lda, lu, rhs, x, c->st);  // x0 = M^{-1} b
+      }
+      *iters = gm.solve(rhs, x, c->tol, c->maxit, c->restart, res);
+      check_status(c);
+      // the target may sit below the FP64 residual floor of an ill-conditioned
+      // system; a stagnated refinement within 1000x of it is accepted
+      if(!(*res <= 1e3 * c->tol))
+        throw OmxError(OMX_ERR_NOCONV, "refinement stopped at relative residual " + std::to_string(*res));
+      return;
+    } catch(OmxError &e) {
+      if(last || (e.code != OMX_ERR_SINGULAR && e.code != OMX_ERR_NOCONV)) throw;
+      ++c->fallbacks;
+      fact = false;
     }
-    gm.prec = [c, A32, &lu, Np, lda](const cplx *in, cplx *out) {
-      omx::lu_solve<float2>(A32, Np, lda, lu, in, out, c->st);
-    };
-    omx::lu_solve<float2>(A32, Np, lda, lu, rhs, x, c->st);  // x0 = M^{-1} b
-  } else {
-    HIPCHK(hipMemsetAsync(x, 0, sizeof(cplx) * Np, c->st));
   }
-  *iters = gm.solve(rhs, x, c->tol, c->maxit, c->restart, res);
-  check_status(c);
-  // the target may sit below the FP64 residual floor of an ill-conditioned
-  // system; a stagnated refinement within 1000x of it is accepted
-  if(*res > 1e3 * c->tol)
-    throw OmxError(OMX_ERR_NOCONV, "iterative solver stopped at relative residual " + std::to_string(*res));
 }
 
 }  // namespace
@@ -473,7 +513,8 @@ void omx_destroy(omx_context *c) {
                   &c->d_S32, &c->d_V64, &c->d_V32, &c->d_x, &c->d_xsh, &c->d_xint, &c->d_xintsh, &c->d_K,
                   &c->d_K1, &c->d_cross, &c->d_cg, &c->d_piv, &c->d_perm, &c->d_cand, &c->d_cnt, &c->d_map,
                   &c->d_wlu, &c->d_status, &c->d_flags, &c->d_basis, &c->d_w1, &c->d_w2, &c->d_r,
-                  &c->d_part, &c->d_h, &c->d_nrm, &c->d_gpart, &c->d_piv2, &c->d_perm2, &c->d_flags2};
+                  &c->d_part, &c->d_h, &c->d_nrm, &c->d_gpart, &c->d_piv2, &c->d_perm2, &c->d_flags2,
+                  &c->d_dinv, &c->d_dinv2, &c->d_F64, &c->d_F64sh};
   for(DBuf *b : bufs) b->release(&c->total);
   for(auto &e : c->ev)
     if(e) (void)hipEventDestroy(e);
@@ -556,6 +597,15 @@ int omx_set_solver(omx_context *c, int kind, double tol, int maxit, int restart)
     c->maxit = maxit;
     c->restart = restart;
     c->updated = c->solved = false;
+  });
+}
+
+int omx_set_pivoting(omx_context *c, int mode) {
+  return guard([&] {
+    if(!c || (mode != OMX_PIVOT_BLOCK && mode != OMX_PIVOT_PARTIAL)) throw OmxError(OMX_ERR_ARG, "invalid pivoting");
+    c->pivot = mode;
+    c->factored = c->factored_sh = false;
+    c->solved = false;
   });
 }
 
@@ -645,6 +695,7 @@ int omx_update(omx_context *c) {
     c->updated = true;
     c->solved = false;
     c->factored = c->factored_sh = false;
+    c->dirty32 = c->dirty32_sh = false;
   });
 }
 
@@ -737,6 +788,7 @@ int omx_get_stats(const omx_context *c, omx_stats *out) {
   out->lu_gemm_launches = c->prof_launches;
   out->t_lu_panel_ms = c->prof_panel_ms;
   out->t_factor_total_ms = c->prof_total_ms;
+  out->pivot_fallbacks = c->fallbacks;
   return OMX_OK;
 }
 
@@ -754,8 +806,6 @@ int omx_read_matrix(omx_context *c, int which, double *out) {
     if(!c || !c->updated || !out) throw OmxError(OMX_ERR_STATE, "no assembled matrix");
     const DBuf &b = which == 0 ? c->d_S64 : c->d_V64;
     if(!b.p) throw OmxError(OMX_ERR_STATE, "matrix not held in FP64");
-    if((which == 0 && c->factored) || (which == 1 && c->factored_sh))
-      throw OmxError(OMX_ERR_STATE, "matrix already factorised");
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpy2DAsync(out, sizeof(cplx) * c->N, b.p, sizeof(cplx) * c->Np, sizeof(cplx) * c->N, c->N,
                             hipMemcpyDeviceToHost, c->st));
@@ -811,6 +861,7 @@ int omx_dense_solve(omx_context *c, int N, const double *A, const double *b, dou
       HIPCHK(hipStreamSynchronize(ctx.st));
     }
     ctx.factored = false;
+    ctx.dirty32 = false;
     int it = 0;
     double res = 0;
     solve_system(&ctx, false, ctx.d_Q.as<cplx>(), ctx.d_x.as<cplx>(), &it, &res);

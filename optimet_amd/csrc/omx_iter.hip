@@ -102,6 +102,14 @@ __global__ void nrm2_reduce_k(const double *part, int nb, double *out) {
 
 static inline dim3 g1(int n) { return dim3((n + 255) / 256); }
 
+// y = (complex float) x over n elements (grid-stride: n may exceed 2^31)
+__global__ void demote_k(float2 *y, const cplx *x, size_t n) {
+  for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] = make_float2((float)x[i].x, (float)x[i].y);
+}
+void demote(float2 *y, const cplx *x, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(demote_k, dim3(4096), dim3(256), 0, st, y, x, n);
+}
 void vcopy(cplx *y, const cplx *x, int n, hipStream_t st) { hipLaunchKernelGGL(vcopy_k, g1(n), dim3(256), 0, st, y, x, n); }
 void vaxpby(cplx *y, cplx a, const cplx *x, cplx b, int n, hipStream_t st) {
   hipLaunchKernelGGL(vaxpby_k, g1(n), dim3(256), 0, st, y, a, x, b, n);

@@ -94,12 +94,18 @@ struct LuWork {
   int *mcnt;
   int *status;          // [0]: singular pivot column+1, [1]: wait timeout
   int *flags;           // Np / NB triangular-solve flags
-  void *wlu;            // 3 x NB x NB complex: selected-block factors, L11^{-1}, U11^{-1}
-  hipStream_t aux;      // look-ahead stream of the panel tournaments
+  void *wlu;            // NB x NB complex: the tournament's selected-block factors
+  void *dinv;           // per panel [L11^{-1} | U11^{-1}], 2 NB^2 elements of T, column-major
+  hipStream_t aux;      // look-ahead stream of the panel pivoting
   hipEvent_t ev_next, ev_piv;
   int gmax;
   int epoch;            // last flag value used by a triangular solve
+  int pivot;            // kPivotBlock or kPivotPartial
 };
+// Pivot search of a panel: among the rows of its diagonal block only
+// (followed by iterative refinement), or over the whole column (tournament).
+constexpr int kPivotBlock = 0;
+constexpr int kPivotPartial = 1;
 
 // Optional per-launch timing of the factorisation (events on the same stream).
 struct LuProf {
@@ -122,6 +128,7 @@ constexpr int kGemvSplitMax = 32;
 
 // level-1 helpers (omx_iter.hip); reductions return through small device arrays
 void vcopy(cplx *y, const cplx *x, int n, hipStream_t st);
+void demote(float2 *y, const cplx *x, size_t n, hipStream_t st);                 // y = (c64) x
 void vaxpby(cplx *y, cplx a, const cplx *x, cplx b, int n, hipStream_t st);   // y = a x + b y
 void vsub(cplx *z, const cplx *x, const cplx *y, int n, hipStream_t st);      // z = x - y
 void vmul_ov(cplx *y, const cplx *x, const cplx *ov, int kind, int P, int nmax, const cplx *sub, int n,

@@ -1,27 +1,31 @@
-// omx_lu.hip -- blocked complex LU with partial pivoting for CDNA4.
+// omx_lu.hip -- blocked complex LU for CDNA4.
 //
 // Right-looking blocked factorisation P A = L U of a column-major Np x Np
 // complex matrix (Np a multiple of the block size NB), in complex FP64
-// (double2) or complex FP32 (float2):
-//   tslu_leaf/merge  panel pivoting by tournament (TSLU): register-resident
-//                  partial-pivoting LUs of 128-row chunks nominate candidate
-//                  rows, a binary tree of merges picks the NB pivot rows; the
-//                  root also emits the LU factors of the selected block
-//   tslu_finish    one wave turns the winners into the swap sequence and a
-//                  gather map
-//   laswp_map      the panel's row interchanges on every column, as one
+// (double2) or complex FP32 (float2).  Two pivot searches:
+//   block (default)  diag_lu: partial pivoting among the rows of the panel's
+//                  diagonal block, in one workgroup, which also inverts L11
+//                  and U11; the solver recovers full accuracy by iterative
+//                  refinement and refactors with the tournament if that fails
+//   partial        tournament pivoting (TSLU): register-resident partial-
+//                  pivoting LUs of 128-row chunks nominate candidate rows, a
+//                  binary tree of merges picks the NB pivot rows (tslu_leaf,
+//                  tslu_merge), tslu_finish turns them into a gather map and
+//                  panel_inv writes the block's factors and inverses
+// Shared by both:
+//   laswp_map      the panel's row interchanges on the other columns, as one
 //                  parallel gather/scatter
-//   panel_inv      writes the selected block's factors and inverts L11, U11
 //   blk_mul        the two panel triangular solves as in-place MFMA block
 //                  products: U12 = L11^{-1} A12, L21 = A21 U11^{-1}
-//   gemm_kernel    A22 -= L21 U12 as complex products on the matrix cores:
+//   gemm_tile      A22 -= L21 U12 as complex products on the matrix cores:
 //                  v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32
 //                  (FP32), four real MFMAs per complex tile product, planar
-//                  re/im operands staged through LDS
+//                  re/im operands staged through LDS; panels are taken in
+//                  pairs so that the bulk update runs with K = 2 NB
 //   trsv_kernel    sync-free blocked triangular solves: workgroup k owns
 //                  block row k, consumes earlier blocks as their flags are
-//                  published and solves its diagonal block in one wave
-// Look-ahead: the tournament of panel k+1 runs on a second stream while the
+//                  published and applies its stored diagonal-block inverse
+// Look-ahead: the pivoting of panel k+1 runs on a second stream while the
 // bulk of panel k's trailing update runs on the first.
 // This replaces the ScaLAPACK pzgesv path (scalapack/LinearSystemSolver.hpp:84-91)
 // and Eigen's ColPivHouseholderQR (PreconditionedMatrixSolver.h:58, 75).
@@ -86,6 +90,23 @@ template <typename T> __device__ inline T crecip(T a) {
   }
   return r;
 }
+
+// Componentwise select (a ternary on the aggregate would go through memory).
+template <typename R> __device__ inline Cx<R> csel(bool p, Cx<R> a, Cx<R> b) {
+  Cx<R> r;
+  r.x = p ? a.x : b.x;
+  r.y = p ? a.y : b.y;
+  return r;
+}
+
+// Value of lane l (wave-uniform) of a complex register pair.
+__device__ inline double rdl1(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline float rdl1(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+template <typename R> __device__ inline Cx<R> rdl(Cx<R> v, int l) { return Cx<R>{rdl1(v.x, l), rdl1(v.y, l)}; }
 
 __global__ void iota_kernel(int *p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -324,7 +345,7 @@ __global__ __launch_bounds__(64) void tslu_finish(const int *sel, const int *nse
 // Writes the selected block's factors into A and inverts L11 (unit lower)
 // and U11 (thread j builds column j of each inverse, one after the other).  wlu holds [factors | Linv | Uinv], each 64 x 64 column-major.
 template <typename T>
-__global__ __launch_bounds__(256) void panel_inv(T *A, size_t lda, int j0, T *wlu, int *status) {
+__global__ __launch_bounds__(256) void panel_inv(T *A, size_t lda, int j0, const T *wlu, T *dinv, int *status) {
   __shared__ T F[64 * 65];
   __shared__ T X[64 * 64];   // row-major: X[i * 64 + j]; L11^{-1}, then U11^{-1}
   const int t = threadIdx.x;
@@ -348,7 +369,7 @@ __global__ __launch_bounds__(256) void panel_inv(T *A, size_t lda, int j0, T *wl
   __syncthreads();
   for(int idx = t; idx < 64 * 64; idx += 256) {
     int c = idx / 64, r = idx % 64;
-    wlu[4096 + idx] = X[r * 64 + c];
+    dinv[idx] = X[r * 64 + c];
   }
   __syncthreads();
   if(t < 64) {  // column j of U11^{-1}
@@ -372,98 +393,106 @@ __global__ __launch_bounds__(256) void panel_inv(T *A, size_t lda, int j0, T *wl
   __syncthreads();
   for(int idx = t; idx < 64 * 64; idx += 256) {
     int c = idx / 64, r = idx % 64;
-    wlu[8192 + idx] = X[r * 64 + c];
+    dinv[4096 + idx] = X[r * 64 + c];
   }
+}
+
+// Maximum of an unsigned key over the 64 lanes of a wave, returned to every
+// lane: row shifts build each 16-lane row's maximum in its lane 15, two row
+// broadcasts combine the rows into lane 63.
+__device__ inline unsigned wave_max_u32(unsigned x) {
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true));  // row_shr:1
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true));  // row_shr:2
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true));  // row_shr:4
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true));  // row_shr:8
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x142, 0xA, 0xF, false));  // row_bcast:15
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x143, 0xC, 0xF, false));  // row_bcast:31
+  return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
 }
 
 // Diagonal-block pivoting: LU with partial pivoting restricted to the 64 x 64
 // diagonal block of the panel (rows are chosen among the block's rows; the
-// interchanges are applied to whole rows), then the inverses of L11 and U11
-// by Gauss-Jordan sweeps.  One workgroup: wave q holds columns 16q..16q+15 of
-// every row, lane = row, and rows are not moved during the elimination (the
-// pivot order is recorded instead).  Writes the factored block into A, the
-// inverses into dinv ([L11^{-1} | U11^{-1}], column-major) and the block's row
-// map (mdst <- msrc) for the remaining columns and the running permutation.
+// interchanges are applied to whole rows), then the inverses of L11 and U11.
+// One workgroup of 8 waves.  Elimination: wave q holds columns 8q..8q+7 of
+// every row in registers, lane = row; rows are not moved (the pivot order is
+// recorded instead).  Per column the owning wave finds the pivot with a DPP
+// maximum of a 32-bit key (max(|re|, |im|) as a float with the row in the
+// low bits) and publishes the column; after one barrier every wave forms the
+// multipliers itself (with the pivot's reciprocal from the owner) and
+// updates its columns with the pivot row read from its own lane b.  Inversion: waves 0-3 build L11^{-1}, waves 4-7 U11^{-1}
+// (lane = row, 16 columns per wave, row k broadcast by lane reads).  Writes
+// the factored block into A, the inverses into dinv ([L11^{-1} | U11^{-1}],
+// column-major), the block's row map (mdst <- msrc) and the permutation.
 template <typename T>
-__global__ __launch_bounds__(256) void diag_lu(T *A, size_t lda, int j0, T *dinv, int *mdst, int *msrc, int *mcnt,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2))) void diag_lu(T *A, size_t lda, int j0, T *dinv, int *mdst, int *msrc, int *mcnt,
                                                int *perm, int *status) {
   using C = CxOf<T>;
-  __shared__ C pr[64];
-  __shared__ C lc[64];
-  __shared__ int pb;
+  __shared__ C colc[2][64];
+  __shared__ C ipc[2];  // reciprocal of the pivot
+  __shared__ int pbs[2];
   __shared__ int prow[64];
   __shared__ C F[64 * 65];  // factored block in pivoted order, column-major [c * 65 + r]
-  __shared__ C X[64 * 64];  // inverse under construction, row-major [i * 64 + j]
   const int t = threadIdx.x, lane = t & 63, q = t >> 6;
-  const C zero{};
-  C v[16];
+  const C zero{}, one{1, 0};
+  {
+    C v[8];
 #pragma unroll
-  for(int j = 0; j < 16; ++j) {
-    T a = A[(size_t)(j0 + 16 * q + j) * lda + j0 + lane];
-    v[j] = C{a.x, a.y};
-  }
-  bool used = false;
-  int mypos = 0;
+    for(int j = 0; j < 8; ++j) {
+      T a = A[(size_t)(j0 + 8 * q + j) * lda + j0 + lane];
+      v[j] = C{a.x, a.y};
+    }
+    bool used = false;
+    int mypos = 0;
 #pragma unroll 1
-  for(int qc = 0; qc < 4; ++qc) {
+    for(int qc = 0; qc < 8; ++qc) {
 #pragma unroll
-    for(int jj = 0; jj < 16; ++jj) {
-      const int c = 16 * qc + jj;
-      if(q == qc) {
-        double val = used ? -1.0 : mag2(v[jj]);
-        int idx = used ? INT_MAX : lane;
-#pragma unroll
-        for(int off = 32; off > 0; off >>= 1) {
-          double ov = __shfl_xor(val, off, 64);
-          int oi = __shfl_xor(idx, off, 64);
-          if(ov > val || (ov == val && oi < idx)) val = ov, idx = oi;
+      for(int jj = 0; jj < 8; ++jj) {
+        const int c = 8 * qc + jj, par = jj & 1;
+        if(q == qc) {
+          const float mx = (float)fmax(fabs((double)v[jj].x), fabs((double)v[jj].y));
+          const unsigned key = used ? 0u : ((__float_as_uint(mx) & ~0x7Fu) | (unsigned)(127 - lane));
+          const unsigned best = wave_max_u32(key);
+          const int bq = 127 - (int)(best & 127);
+          colc[par][lane] = v[jj];
+          if(lane == bq) {
+            const bool sing = !(mag2(v[jj]) > 0.0);
+            if(sing) atomicCAS(&status[0], 0, j0 + c + 1);
+            pbs[par] = bq;
+            ipc[par] = csel(sing, zero, crecip(v[jj]));
+          }
         }
-        if(lane == 0) {
-          pb = idx;
-          prow[c] = idx;
+        __syncthreads();
+        const int b = __builtin_amdgcn_readfirstlane(pbs[par]);
+        const C cv = colc[par][lane];
+        const C ip = ipc[par];
+        if(lane == b) {
+          used = true;
+          mypos = c;
         }
-      }
-      __syncthreads();
-      const int b = pb;
-      if(lane == b) {
+        if(t == 0) prow[c] = b;
+        const C l = csel(used, zero, cm(cv, ip));
+        if(q == qc) {
+          v[jj] = csel(used, v[jj], l);
 #pragma unroll
-        for(int j = 0; j < 16; ++j) pr[16 * q + j] = v[j];
-        used = true;
-        mypos = c;
-      }
-      if(q == qc) {
-        C piv;
-        piv.x = __shfl(v[jj].x, b, 64);
-        piv.y = __shfl(v[jj].y, b, 64);
-        const bool sing = !(mag2(piv) > 0.0);
-        if(sing && lane == 0) atomicCAS(&status[0], 0, j0 + c + 1);
-        const C ip = sing ? zero : crecip(piv);
-        C l = zero;
-        if(!used) {
-          l = cm(v[jj], ip);
-          v[jj] = l;
+          for(int j = jj + 1; j < 8; ++j) v[j] = csub(v[j], cm(l, rdl(v[j], b)));
+        } else if(q > qc) {
+#pragma unroll
+          for(int j = 0; j < 8; ++j) v[j] = csub(v[j], cm(l, rdl(v[j], b)));
         }
-        lc[lane] = l;
-      }
-      __syncthreads();
-      if(!used) {
-        const C l = (q == qc) ? v[jj] : lc[lane];
-#pragma unroll
-        for(int j = 0; j < 16; ++j)
-          if(16 * q + j > c) v[j] = csub(v[j], cm(l, pr[16 * q + j]));
       }
     }
-  }
-  // factored block in pivoted order
+    // factored block in pivoted order
 #pragma unroll
-  for(int j = 0; j < 16; ++j) {
-    const int col = 16 * q + j;
-    F[col * 65 + mypos] = v[j];
-    T o;
-    o.x = v[j].x;
-    o.y = v[j].y;
-    A[(size_t)(j0 + col) * lda + j0 + mypos] = o;
+    for(int j = 0; j < 8; ++j) {
+      const int col = 8 * q + j;
+      F[col * 65 + mypos] = v[j];
+      T o;
+      o.x = v[j].x;
+      o.y = v[j].y;
+      A[(size_t)(j0 + col) * lda + j0 + mypos] = o;
+    }
   }
+  __syncthreads();
   // row map and running permutation (wave 0)
   if(q == 0) {
     const int src = prow[lane];
@@ -480,63 +509,45 @@ __global__ __launch_bounds__(256) void diag_lu(T *A, size_t lda, int j0, T *dinv
     __builtin_amdgcn_wave_barrier();
     perm[j0 + lane] = old;
   }
-  __syncthreads();
-  // L11^{-1}: X = I; for k: X[i][j] -= L[i][k] X[k][j], i > k, j <= k
-  const int i = t & 63, jb = t >> 6;
-  for(int m = 0; m < 16; ++m) {
-    const int j = jb + 4 * m;
-    X[i * 64 + j] = (i == j) ? C{1, 0} : zero;
-  }
-  __syncthreads();
-  for(int k = 0; k < 63; ++k) {
-    if(i > k) {
-      const C lik = F[k * 65 + i];
-      for(int m = 0; m < 16; ++m) {
-        const int j = jb + 4 * m;
-        if(j <= k) X[i * 64 + j] = csub(X[i * 64 + j], cm(lik, X[k * 64 + j]));
-      }
+  // Inversions: lane = row i; waves 0-3 hold columns 16 w .. 16 w + 15 of
+  // L11^{-1}, waves 4-7 those of U11^{-1}; row k is broadcast within the wave
+  // by lane reads, so the waves run without barriers.
+  const int i = lane, w = q & 3;
+  C x[16];
+#pragma unroll
+  for(int m = 0; m < 16; ++m) x[m] = csel(16 * w + m == i, one, zero);
+  if(q < 4) {
+    // L11^{-1} (unit lower): for k: row i -= L[i][k] row k, i > k
+#pragma unroll 1
+    for(int k = 0; k < 63; ++k) {
+      const int ks = __builtin_amdgcn_readfirstlane(k);
+      const C lik = csel(i > ks, F[ks * 65 + i], zero);
+#pragma unroll
+      for(int m = 0; m < 16; ++m) x[m] = csub(x[m], cm(lik, rdl(x[m], ks)));
     }
-    __syncthreads();
-  }
-  for(int m = 0; m < 16; ++m) {
-    const int j = jb + 4 * m;
-    const C x = X[i * 64 + j];
-    T o;
-    o.x = x.x;
-    o.y = x.y;
-    dinv[j * 64 + i] = o;
-  }
-  __syncthreads();
-  // U11^{-1}: Y = I; for k = 63..0: Y[k][:] /= U[k][k]; Y[i][:] -= U[i][k] Y[k][:], i < k
-  for(int m = 0; m < 16; ++m) {
-    const int j = jb + 4 * m;
-    X[i * 64 + j] = (i == j) ? C{1, 0} : zero;
-  }
-  __syncthreads();
-  for(int k = 63; k >= 0; --k) {
-    const C d = F[k * 65 + k];
-    const C id = mag2(d) > 0.0 ? crecip(d) : zero;
-    C yk[16];
-    for(int m = 0; m < 16; ++m) yk[m] = cm(X[k * 64 + jb + 4 * m], id);
-    __syncthreads();
-    if(i == k) {
-      for(int m = 0; m < 16; ++m) X[k * 64 + jb + 4 * m] = yk[m];
-    } else if(i < k) {
-      const C uik = F[k * 65 + i];
-      for(int m = 0; m < 16; ++m) {
-        const int j = jb + 4 * m;
-        if(j >= k) X[i * 64 + j] = csub(X[i * 64 + j], cm(uik, yk[m]));
-      }
+  } else {
+    // U11^{-1} = (D^{-1} U11)^{-1} D^{-1} with D = diag(U11): a unit-upper
+    // sweep (row i -= (U[i][k] / U[i][i]) row k, i < k, k = 63..0), then
+    // column j scaled by 1 / U[j][j]
+    const C di = F[i * 65 + i];
+    const C rinv = csel(mag2(di) > 0.0, crecip(di), zero);
+#pragma unroll 1
+    for(int k = 63; k > 0; --k) {
+      const int ks = __builtin_amdgcn_readfirstlane(k);
+      const C uik = csel(i < ks, cm(F[ks * 65 + i], rinv), zero);
+#pragma unroll
+      for(int m = 0; m < 16; ++m) x[m] = csub(x[m], cm(uik, rdl(x[m], ks)));
     }
-    __syncthreads();
+#pragma unroll
+    for(int m = 0; m < 16; ++m) x[m] = cm(x[m], rdl(rinv, 16 * w + m));
   }
+  T *out = dinv + (q < 4 ? 0 : 4096);
+#pragma unroll
   for(int m = 0; m < 16; ++m) {
-    const int j = jb + 4 * m;
-    const C x = X[i * 64 + j];
     T o;
-    o.x = x.x;
-    o.y = x.y;
-    dinv[4096 + j * 64 + i] = o;
+    o.x = x[m].x;
+    o.y = x[m].y;
+    out[(16 * w + m) * 64 + i] = o;
   }
 }
 
@@ -623,12 +634,15 @@ __global__ __launch_bounds__(256) void blk_mul(T *A, size_t lda, int j0, int fir
 // ------------------------------------------------------------------ swaps
 // Applies the panel's gather map to every column: each workgroup handles 8
 // columns, reads all moved entries into registers, then writes them back.
+// Columns skip0 .. skip0 + NB - 1 are left out when skip0 >= 0.
 template <typename T>
 __global__ __launch_bounds__(256) void laswp_map(T *A, size_t lda, int Np, const int *__restrict__ mdst,
-                                                 const int *__restrict__ msrc, const int *__restrict__ mcnt) {
+                                                 const int *__restrict__ msrc, const int *__restrict__ mcnt,
+                                                 int skip0) {
   const int m = *mcnt;
   const int cl = threadIdx.x / 32, li = threadIdx.x % 32;
-  const int col = blockIdx.x * 8 + cl;
+  int col = blockIdx.x * 8 + cl;
+  if(skip0 >= 0 && col >= skip0) col += kNB;
   T v[4];
   int d[4];
   if(col < Np) {
@@ -650,146 +664,258 @@ __global__ __launch_bounds__(256) void laswp_map(T *A, size_t lda, int Np, const
 }
 
 // ------------------------------------------------------------------ gemm
-// C[row0 + m, col0 + n] -= sum_{k < NB} A[row0 + m, k0 + k] A[k0 + k, col0 + n]
-// Tile 64 x 64 per workgroup, four waves in 2 x 2, each wave 2 x 2 MFMA
-// 16 x 16 blocks with separate real and imaginary accumulators.
-template <typename T, int NB>
-__global__ __launch_bounds__(256) void gemm_kernel(T *A, size_t lda, int row0, int col0, int k0) {
+// Trailing update:
+// C[row0 + m, col0 + n] -= sum_{k < KD} A[row0 + m, k0 + k] A[k0 + k, col0 + n]
+// for m < M, n < N.  A TM x TN tile per workgroup of WGM x WGN waves, each
+// wave (TM/WGM) x (TN/WGN) as 16 x 16 MFMA blocks with separate real and
+// imaginary accumulators; K is staged through LDS in chunks of BK (planar
+// re/im, rows padded to 16 banks) and the next chunk is fetched into
+// registers while the current one is multiplied.
+template <typename T, int TM, int TN, int KD, int WGM, int WGN, int BK>
+__global__ __launch_bounds__(64 * WGM * WGN) void gemm_tile(T *A, size_t lda, int row0, int col0, int k0, int M,
+                                                           int N) {
   using R = typename CT<T>::R;
   using V4 = typename CT<T>::V4;
-  constexpr int BK = 16, PAD = 4;
-  __shared__ R As[2][BK][64 + PAD];
-  __shared__ R Bs[2][BK][64 + PAD];
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, wm = wv & 1, wn = wv >> 1;
-  const int bm = row0 + blockIdx.x * 64, bn = col0 + blockIdx.y * 64;
-  V4 cr[2][2], ci[2][2];
+  constexpr int NT = 64 * WGM * WGN, PAD = 16;
+  constexpr int WM = TM / WGM, WN = TN / WGN, FM = WM / 16, FN = WN / 16;
+  constexpr int LA = TM * BK / NT, LB = TN * BK / NT;
+  static_assert(LA * NT == TM * BK && LB * NT == TN * BK, "tile not divisible");
+  __shared__ R As[2][BK][TM + PAD];
+  __shared__ R Bs[2][BK][TN + PAD];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, wm = wv % WGM, wn = wv / WGM;
+  const int bm = blockIdx.x * TM, bn = blockIdx.y * TN;
+  V4 cr[FM][FN], ci[FM][FN];
 #pragma unroll
-  for(int a = 0; a < 2; ++a)
+  for(int a = 0; a < FM; ++a)
 #pragma unroll
-    for(int b = 0; b < 2; ++b) {
+    for(int b = 0; b < FN; ++b) {
       cr[a][b] = V4{0, 0, 0, 0};
       ci[a][b] = V4{0, 0, 0, 0};
     }
-  for(int kc = 0; kc < NB; kc += BK) {
+  R ar_[LA], ai_[LA], br_[LB], bi_[LB];
+  auto gload = [&](int kc) {
 #pragma unroll
-    for(int it = 0; it < (64 * BK) / 256; ++it) {
-      int idx = t + it * 256;
-      int r = idx & 63, kk = idx >> 6;
-      T v = A[(size_t)(k0 + kc + kk) * lda + bm + r];
-      As[0][kk][r] = v.x;
-      As[1][kk][r] = v.y;
-      int kb = idx % BK, c = idx / BK;
-      T wv2 = A[(size_t)(bn + c) * lda + k0 + kc + kb];
-      Bs[0][kb][c] = wv2.x;
-      Bs[1][kb][c] = wv2.y;
+    for(int i = 0; i < LA; ++i) {
+      const int idx = t + NT * i, r = idx % TM, kk = idx / TM;
+      ar_[i] = 0;
+      ai_[i] = 0;
+      if(bm + r < M) {
+        const T v = A[(size_t)(k0 + kc + kk) * lda + row0 + bm + r];
+        ar_[i] = v.x;
+        ai_[i] = v.y;
+      }
+    }
+#pragma unroll
+    for(int i = 0; i < LB; ++i) {
+      const int idx = t + NT * i, kb = idx % BK, c = idx / BK;
+      br_[i] = 0;
+      bi_[i] = 0;
+      if(bn + c < N) {
+        const T v = A[(size_t)(col0 + bn + c) * lda + k0 + kc + kb];
+        br_[i] = v.x;
+        bi_[i] = v.y;
+      }
+    }
+  };
+  gload(0);
+#pragma unroll 1
+  for(int kc = 0; kc < KD; kc += BK) {
+    __syncthreads();  // the previous chunk has been read out of LDS
+#pragma unroll
+    for(int i = 0; i < LA; ++i) {
+      const int idx = t + NT * i, r = idx % TM, kk = idx / TM;
+      As[0][kk][r] = ar_[i];
+      As[1][kk][r] = ai_[i];
+    }
+#pragma unroll
+    for(int i = 0; i < LB; ++i) {
+      const int idx = t + NT * i, kb = idx % BK, c = idx / BK;
+      Bs[0][kb][c] = br_[i];
+      Bs[1][kb][c] = bi_[i];
     }
     __syncthreads();
+    if(kc + BK < KD) gload(kc + BK);
 #pragma unroll
     for(int ks = 0; ks < BK / 4; ++ks) {
       const int kr = ks * 4 + (lane >> 4), li = lane & 15;
-      R ar[2], ai[2], nai[2], br[2], bi[2];
+      R xr[FM], xi[FM], nxi[FM], yr[FN], yi[FN];
 #pragma unroll
-      for(int a = 0; a < 2; ++a) {
-        ar[a] = As[0][kr][wm * 32 + a * 16 + li];
-        ai[a] = As[1][kr][wm * 32 + a * 16 + li];
-        nai[a] = -ai[a];
-        br[a] = Bs[0][kr][wn * 32 + a * 16 + li];
-        bi[a] = Bs[1][kr][wn * 32 + a * 16 + li];
+      for(int a = 0; a < FM; ++a) {
+        xr[a] = As[0][kr][wm * WM + a * 16 + li];
+        xi[a] = As[1][kr][wm * WM + a * 16 + li];
+        nxi[a] = -xi[a];
       }
 #pragma unroll
-      for(int a = 0; a < 2; ++a)
+      for(int b = 0; b < FN; ++b) {
+        yr[b] = Bs[0][kr][wn * WN + b * 16 + li];
+        yi[b] = Bs[1][kr][wn * WN + b * 16 + li];
+      }
 #pragma unroll
-        for(int b = 0; b < 2; ++b) {
-          cr[a][b] = CT<T>::mfma(ar[a], br[b], cr[a][b]);
-          cr[a][b] = CT<T>::mfma(nai[a], bi[b], cr[a][b]);
-          ci[a][b] = CT<T>::mfma(ar[a], bi[b], ci[a][b]);
-          ci[a][b] = CT<T>::mfma(ai[a], br[b], ci[a][b]);
-        }
+      for(int a = 0; a < FM; ++a)
+#pragma unroll
+        for(int b = 0; b < FN; ++b) cr[a][b] = CT<T>::mfma(xr[a], yr[b], cr[a][b]);
+#pragma unroll
+      for(int a = 0; a < FM; ++a)
+#pragma unroll
+        for(int b = 0; b < FN; ++b) ci[a][b] = CT<T>::mfma(xr[a], yi[b], ci[a][b]);
+#pragma unroll
+      for(int a = 0; a < FM; ++a)
+#pragma unroll
+        for(int b = 0; b < FN; ++b) cr[a][b] = CT<T>::mfma(nxi[a], yi[b], cr[a][b]);
+#pragma unroll
+      for(int a = 0; a < FM; ++a)
+#pragma unroll
+        for(int b = 0; b < FN; ++b) ci[a][b] = CT<T>::mfma(xi[a], yr[b], ci[a][b]);
     }
-    __syncthreads();
   }
 #pragma unroll
-  for(int a = 0; a < 2; ++a)
+  for(int a = 0; a < FM; ++a)
 #pragma unroll
-    for(int b = 0; b < 2; ++b) {
-      int col = bn + wn * 32 + b * 16 + (lane & 15);
-      int rowb = bm + wm * 32 + a * 16;
-      T *cp = A + (size_t)col * lda + rowb;
+    for(int b = 0; b < FN; ++b) {
+      const int n = bn + wn * WN + b * 16 + (lane & 15);
+      if(n >= N) continue;
+      T *cp = A + (size_t)(col0 + n) * lda + row0;
 #pragma unroll
       for(int v = 0; v < 4; ++v) {
-        int rr = CT<T>::row(lane, v);
-        T c = cp[rr];
-        c.x -= cr[a][b][v];
-        c.y -= ci[a][b][v];
-        cp[rr] = c;
+        const int m = bm + wm * WM + a * 16 + CT<T>::row(lane, v);
+        if(m < M) {
+          T c = cp[m];
+          c.x -= cr[a][b][v];
+          c.y -= ci[a][b][v];
+          cp[m] = c;
+        }
       }
     }
 }
 
+// Tile configuration of the trailing updates.
+template <typename T> struct UpdTile {
+  static constexpr int TM = 64, TN = 64, WGM = 2, WGN = 2, BK = 16;
+};
+
+// C -= A B over an M x N block (M, N multiples of 64) with K = KD.
+template <typename T, int KD>
+void trailing_update(T *A, size_t lda, int row0, int col0, int k0, int M, int N, hipStream_t st) {
+  if(M <= 0 || N <= 0) return;
+  using U = UpdTile<T>;
+  hipLaunchKernelGGL((gemm_tile<T, U::TM, U::TN, KD, U::WGM, U::WGN, U::BK>),
+                     dim3((M + U::TM - 1) / U::TM, (N + U::TN - 1) / U::TN), dim3(64 * U::WGM * U::WGN), 0, st, A, lda,
+                     row0, col0, k0, M, N);
+}
+
 // ------------------------------------------------------------------ driver
+// Panels of NB columns, taken in pairs (p0, p1) so that the bulk of the
+// trailing update runs with K = 2 NB.  Per pair, on the main stream st:
+//   panel p0      wait for its pivoting (issued earlier on the look-ahead
+//                 stream sb), laswp_map, [panel_inv], blk_mul L21 and U12
+//   update        p1's columns with p0 (K = NB); then block row p1 beyond
+//                 p1 with p0 (K = NB): with block pivoting while sb pivots
+//                 p1, with the tournament after p1's interchanges
+//   panel p1      as p0
+//   bulk update   A22 -= [L21(p0) L21(p1)] [U12(p0); U12(p1)] (K = 2 NB): the
+//                 next pair's first columns first, then, while sb pivots that
+//                 panel, the rest
 template <typename T>
 int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof) {
   if(Np % kNB) return -1;
+  const bool block = w.pivot == kPivotBlock;
   hipStream_t sb = w.aux;
   T *wlu = reinterpret_cast<T *>(w.wlu);
+  T *dinv = reinterpret_cast<T *>(w.dinv);
   hipLaunchKernelGGL(iota_kernel, dim3((Np + 255) / 256), dim3(256), 0, st, w.perm, Np);
   if(prof) {
     prof->used = 0;
     prof->gemm_flop = 0.0;
   }
-  auto tournament = [&](int j0) {
-    const int G = (Np - j0 + kPanelRows - 1) / kPanelRows;
-    hipLaunchKernelGGL((tslu_leaf<T>), dim3(G), dim3(256), 0, sb, A, lda, Np, j0, w.cand[0], w.cnt[0], wlu);
-    int gin = G, buf = 0;
-    while(gin > 1) {
-      const int gout = (gin + 1) / 2;
-      hipLaunchKernelGGL((tslu_merge<T>), dim3(gout), dim3(256), 0, sb, A, lda, j0, gin, w.cand[buf], w.cnt[buf],
-                         w.cand[buf ^ 1], w.cnt[buf ^ 1], wlu);
-      buf ^= 1;
-      gin = gout;
+  auto pivoting = [&](int j0) {
+    (void)hipEventRecord(w.ev_next, st);
+    (void)hipStreamWaitEvent(sb, w.ev_next, 0);
+    if(block) {
+      hipLaunchKernelGGL((diag_lu<T>), dim3(1), dim3(512), 0, sb, A, lda, j0, dinv + (size_t)(j0 / kNB) * 2 * kNB * kNB,
+                         w.mdst, w.msrc, w.mcnt, w.perm, w.status);
+    } else {
+      const int G = (Np - j0 + kPanelRows - 1) / kPanelRows;
+      hipLaunchKernelGGL((tslu_leaf<T>), dim3(G), dim3(256), 0, sb, A, lda, Np, j0, w.cand[0], w.cnt[0], wlu);
+      int gin = G, buf = 0;
+      while(gin > 1) {
+        const int gout = (gin + 1) / 2;
+        hipLaunchKernelGGL((tslu_merge<T>), dim3(gout), dim3(256), 0, sb, A, lda, j0, gin, w.cand[buf], w.cnt[buf],
+                           w.cand[buf ^ 1], w.cnt[buf ^ 1], wlu);
+        buf ^= 1;
+        gin = gout;
+      }
+      hipLaunchKernelGGL(tslu_finish, dim3(1), dim3(64), 0, sb, w.cand[buf], w.cnt[buf], j0, w.piv, w.mdst, w.msrc,
+                         w.mcnt, w.perm);
     }
-    hipLaunchKernelGGL(tslu_finish, dim3(1), dim3(64), 0, sb, w.cand[buf], w.cnt[buf], j0, w.piv, w.mdst, w.msrc,
-                       w.mcnt, w.perm);
     (void)hipEventRecord(w.ev_piv, sb);
   };
-  if((Np - 0 + kPanelRows - 1) / kPanelRows > w.gmax) return -2;
-  (void)hipEventRecord(w.ev_next, st);
-  (void)hipStreamWaitEvent(sb, w.ev_next, 0);
-  tournament(0);
-  for(int j0 = 0; j0 < Np; j0 += kNB) {
-    const int pi = j0 / kNB;
-    const bool rec = prof && pi < prof->cap;
-    const int rest = Np - j0 - kNB;
+  // the pivoting's row interchanges and (tournament) the block's factors
+  auto swaps = [&](int j0) {
+    T *dk = dinv + (size_t)(j0 / kNB) * 2 * kNB * kNB;
     (void)hipStreamWaitEvent(st, w.ev_piv, 0);
-    if(rec) (void)hipEventRecord(prof->ev[4 * pi], st);
-    hipLaunchKernelGGL((laswp_map<T>), dim3((Np + 7) / 8), dim3(256), 0, st, A, lda, Np, w.mdst, w.msrc, w.mcnt);
-    hipLaunchKernelGGL((panel_inv<T>), dim3(1), dim3(256), 0, st, A, lda, j0, wlu, w.status);
+    if(block) {
+      // the diagonal block was factored in pivoted order by diag_lu
+      if(Np > kNB)
+        hipLaunchKernelGGL((laswp_map<T>), dim3((Np - kNB + 7) / 8), dim3(256), 0, st, A, lda, Np, w.mdst, w.msrc,
+                           w.mcnt, j0);
+    } else {
+      hipLaunchKernelGGL((laswp_map<T>), dim3((Np + 7) / 8), dim3(256), 0, st, A, lda, Np, w.mdst, w.msrc, w.mcnt,
+                         -1);
+      hipLaunchKernelGGL((panel_inv<T>), dim3(1), dim3(256), 0, st, A, lda, j0, wlu, dk, w.status);
+    }
+  };
+  // L21 and U12 of panel j0 (after its swaps)
+  auto solves = [&](int j0) {
+    const int rest = Np - j0 - kNB;
+    T *dk = dinv + (size_t)(j0 / kNB) * 2 * kNB * kNB;
     if(rest > 0) {
+      hipLaunchKernelGGL((blk_mul<T, true>), dim3(rest / 64), dim3(256), 0, st, A, lda, j0, j0 + kNB, dk);
       hipLaunchKernelGGL((blk_mul<T, false>), dim3(rest / 64), dim3(256), 0, st, A, lda, j0, j0 + kNB,
-                         wlu + 8192);
-      hipLaunchKernelGGL((blk_mul<T, true>), dim3(rest / 64), dim3(256), 0, st, A, lda, j0, j0 + kNB, wlu + 4096);
+                         dk + kNB * kNB);
     }
-    if(rec) (void)hipEventRecord(prof->ev[4 * pi + 1], st);
-    if(rest > 0) {
-      if(rec) (void)hipEventRecord(prof->ev[4 * pi + 2], st);
-      // next panel's columns first, then its tournament overlaps the rest
-      hipLaunchKernelGGL((gemm_kernel<T, kNB>), dim3(rest / 64, 1), dim3(256), 0, st, A, lda, j0 + kNB, j0 + kNB,
-                         j0);
-      (void)hipEventRecord(w.ev_next, st);
-      (void)hipStreamWaitEvent(sb, w.ev_next, 0);
-      tournament(j0 + kNB);
-      if(rest > 64)
-        hipLaunchKernelGGL((gemm_kernel<T, kNB>), dim3(rest / 64, rest / 64 - 1), dim3(256), 0, st, A, lda,
-                           j0 + kNB, j0 + 2 * kNB, j0);
-      if(rec) {
-        (void)hipEventRecord(prof->ev[4 * pi + 3], st);
-        prof->gemm_flop += 8.0 * (double)rest * rest * kNB;
+  };
+  auto panel = [&](int j0) {
+    swaps(j0);
+    solves(j0);
+  };
+  if(!block && (Np + kPanelRows - 1) / kPanelRows > w.gmax) return -2;
+  pivoting(0);
+  for(int J = 0, pi = 0; J < Np; J += 2 * kNB, ++pi) {
+    const int p0 = J, p1 = J + kNB;
+    const bool rec = prof && pi < prof->cap;
+    hipEvent_t *pe = rec ? prof->ev + 4 * pi : nullptr;
+    if(rec) (void)hipEventRecord(pe[0], st);
+    panel(p0);
+    const int rest1 = Np - p1 - kNB;  // beyond the pair
+    if(p1 < Np) {
+      trailing_update<T, kNB>(A, lda, p1, p1, p0, Np - p1, kNB, st);
+      pivoting(p1);
+      if(block) {
+        // interchanges stay inside block row p1, so its update with p0 can
+        // run while p1 is pivoted
+        trailing_update<T, kNB>(A, lda, p1, p1 + kNB, p0, kNB, rest1, st);
+        panel(p1);
+      } else {
+        // rows from below (not yet updated with p0 beyond p1) may move into
+        // block row p1: interchange first, then update the block row
+        swaps(p1);
+        trailing_update<T, kNB>(A, lda, p1, p1 + kNB, p0, kNB, rest1, st);
+        solves(p1);
       }
-    } else if(rec) {
-      (void)hipEventRecord(prof->ev[4 * pi + 2], st);
-      (void)hipEventRecord(prof->ev[4 * pi + 3], st);
     }
-    if(rec) prof->used = pi + 1;
+    if(rec) (void)hipEventRecord(pe[1], st);
+    if(rec) (void)hipEventRecord(pe[2], st);
+    if(p1 < Np && rest1 > 0) {
+      const int Jn = p1 + kNB;
+      trailing_update<T, 2 * kNB>(A, lda, Jn, Jn, p0, rest1, kNB, st);
+      pivoting(Jn);
+      trailing_update<T, 2 * kNB>(A, lda, Jn, Jn + kNB, p0, rest1, rest1 - kNB, st);
+      if(rec) prof->gemm_flop += 8.0 * (double)rest1 * rest1 * (2 * kNB);
+    }
+    if(rec) {
+      (void)hipEventRecord(pe[3], st);
+      prof->used = pi + 1;
+    }
   }
   return 0;
 }
@@ -800,21 +926,25 @@ __global__ void gather_kernel(const cplx *b, const int *perm, cplx *y, int Np) {
   if(i < Np) y[i] = b[perm[i]];
 }
 
+// Sync-free blocked triangular solve.  Workgroup k owns block row k: it
+// accumulates the products of its off-diagonal tiles with the blocks x_b as
+// their flags are published, then applies the stored inverse of its diagonal
+// block (unit-lower L11^{-1} or U11^{-1}) as a 64 x 64 matrix-vector product.
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size_t lda, int Np, cplx *x, int *flags,
-                                                   int epoch, int *status) {
+__global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size_t lda, int Np,
+                                                   const T *__restrict__ dinv, cplx *x, int *flags, int epoch,
+                                                   int *status) {
   constexpr int NB = kNB;
   const int nblk = Np / NB;
   const int k = LOWER ? blockIdx.x : nblk - 1 - blockIdx.x;
   __shared__ cplx xs[NB];
   __shared__ cplx part[4][NB];
-  __shared__ T dg[NB * (NB + 1)];  // diagonal block, column-major, padded
   const int t = threadIdx.x, r = t & 63, cq = t >> 6;
-  // the diagonal block does not depend on other workgroups: stage it first
-  for(int idx = t; idx < NB * NB; idx += 256) {
-    int c = idx / NB, rr = idx % NB;
-    dg[c * (NB + 1) + rr] = A[(size_t)(k * NB + c) * lda + (size_t)k * NB + rr];
-  }
+  // this thread's 16 entries of the inverse diagonal block: row r, columns 16 cq ..
+  const T *mi = dinv + (size_t)k * 2 * NB * NB + (LOWER ? 0 : NB * NB);
+  T mv[16];
+#pragma unroll
+  for(int cc = 0; cc < 16; ++cc) mv[cc] = mi[(size_t)(cq * 16 + cc) * NB + r];
   cplx acc = mk(0.0);
   const int ndep = LOWER ? k : nblk - 1 - k;
   for(int ii = 0; ii < ndep; ++ii) {
@@ -856,37 +986,19 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
   }
   part[cq][r] = acc;
   __syncthreads();
-  if(t < NB) {
-    cplx s = x[(size_t)k * NB + t] - (part[0][t] + part[1][t] + part[2][t] + part[3][t]);
-    if(LOWER) {
-      for(int i = 0; i < NB; ++i) {
-        cplx yi;
-        yi.x = __shfl(s.x, i, 64);
-        yi.y = __shfl(s.y, i, 64);
-        if(t > i) {
-          T v = dg[i * (NB + 1) + t];
-          s.x -= (double)v.x * yi.x - (double)v.y * yi.y;
-          s.y -= (double)v.x * yi.y + (double)v.y * yi.x;
-        }
-      }
-    } else {
-      for(int i = NB - 1; i >= 0; --i) {
-        if(t == i) {
-          T v = dg[i * (NB + 1) + i];
-          s = s / mk((double)v.x, (double)v.y);
-        }
-        cplx yi;
-        yi.x = __shfl(s.x, i, 64);
-        yi.y = __shfl(s.y, i, 64);
-        if(t < i) {
-          T v = dg[i * (NB + 1) + t];
-          s.x -= (double)v.x * yi.x - (double)v.y * yi.y;
-          s.y -= (double)v.x * yi.y + (double)v.y * yi.x;
-        }
-      }
-    }
-    x[(size_t)k * NB + t] = s;
+  if(t < NB) xs[t] = x[(size_t)k * NB + t] - (part[0][t] + part[1][t] + part[2][t] + part[3][t]);
+  __syncthreads();
+  cplx y = mk(0.0);
+#pragma unroll
+  for(int cc = 0; cc < 16; ++cc) {
+    T v = mv[cc];
+    cplx sv = xs[cq * 16 + cc];
+    y.x += (double)v.x * sv.x - (double)v.y * sv.y;
+    y.y += (double)v.x * sv.y + (double)v.y * sv.x;
   }
+  part[cq][r] = y;
+  __syncthreads();
+  if(t < NB) x[(size_t)k * NB + t] = part[0][t] + part[1][t] + part[2][t] + part[3][t];
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if(t == 0) __hip_atomic_store(&flags[k], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -894,12 +1006,14 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
 
 template <typename T>
 void lu_solve(const T *A, int Np, size_t lda, LuWork &w, const cplx *b, cplx *x, hipStream_t st) {
+  const T *dinv = reinterpret_cast<const T *>(w.dinv);
   hipLaunchKernelGGL(gather_kernel, dim3((Np + 255) / 256), dim3(256), 0, st, b, w.perm, x, Np);
   int nblk = Np / kNB;
   int e1 = ++w.epoch;
-  hipLaunchKernelGGL((trsv_kernel<T, true>), dim3(nblk), dim3(256), 0, st, A, lda, Np, x, w.flags, e1, w.status);
+  hipLaunchKernelGGL((trsv_kernel<T, true>), dim3(nblk), dim3(256), 0, st, A, lda, Np, dinv, x, w.flags, e1,
+                     w.status);
   int e2 = ++w.epoch;
-  hipLaunchKernelGGL((trsv_kernel<T, false>), dim3(nblk), dim3(256), 0, st, A, lda, Np, x, w.flags, e2,
+  hipLaunchKernelGGL((trsv_kernel<T, false>), dim3(nblk), dim3(256), 0, st, A, lda, Np, dinv, x, w.flags, e2,
                      w.status);
 }
 

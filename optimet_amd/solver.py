@@ -16,6 +16,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboptimet_mi355x.so")
 
 LU64, LU32_IR, GMRES64 = 0, 1, 2
+PIVOT_BLOCK, PIVOT_PARTIAL = 0, 1
+PIVOTS = {"block": PIVOT_BLOCK, "partial": PIVOT_PARTIAL}
 KINDS = {"lu64": LU64, "lu32ir": LU32_IR, "gmres64": GMRES64}
 
 ERRORS = {-1: "argument", -2: "device", -3: "memory", -4: "singular", -5: "state", -6: "no convergence",
@@ -24,7 +26,7 @@ ERRORS = {-1: "argument", -2: "device", -3: "memory", -4: "singular", -5: "state
 EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "omx_set_excitation",
            "omx_set_solver", "omx_set_cg_tables", "omx_build_cg_tables", "omx_update", "omx_solve",
            "omx_cross_sections", "omx_scattering_size", "omx_get_stats", "omx_read_matrix", "omx_read_vector",
-           "omx_dense_solve", "omx_set_profile"]
+           "omx_dense_solve", "omx_set_profile", "omx_set_pivoting"]
 
 
 class OmxError(RuntimeError):
@@ -40,7 +42,8 @@ class Stats(ctypes.Structure):
                 ("t_sh_source_ms", ctypes.c_double), ("t_assemble_ms", ctypes.c_double),
                 ("device_bytes", ctypes.c_size_t), ("t_lu_gemm_ms", ctypes.c_double),
                 ("lu_gemm_flop", ctypes.c_double), ("lu_gemm_launches", ctypes.c_int),
-                ("t_lu_panel_ms", ctypes.c_double), ("t_factor_total_ms", ctypes.c_double)]
+                ("t_lu_panel_ms", ctypes.c_double), ("t_factor_total_ms", ctypes.c_double),
+                ("pivot_fallbacks", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -76,6 +79,7 @@ def load_library(path=LIB_PATH):
     L.omx_read_vector.argtypes = [P, I, D]
     L.omx_dense_solve.argtypes = [P, I, D, D, D, I]
     L.omx_set_profile.argtypes = [P, I]
+    L.omx_set_pivoting.argtypes = [P, I]
     _lib = L
     return L
 
@@ -121,6 +125,10 @@ class Context:
         _check(self.L.omx_get_stats(self.h, ctypes.byref(s)))
         return s.as_dict()
 
+    def set_pivoting(self, mode):
+        """"block" (default) or "partial" (OMX_PIVOT_*)."""
+        _check(self.L.omx_set_pivoting(self.h, PIVOTS[mode]))
+
     def dense_solve(self, A, b, kind="lu64"):
         A = np.asfortranarray(A, dtype=np.complex128)
         N = A.shape[0]
@@ -136,13 +144,16 @@ class Solver:
     """MI355X backend behind the AbstractSolver interface (Solver.h:62-144).
 
     kind: "lu64" (FP64 LU), "lu32ir" (complex-FP32 MFMA LU + FP64 GMRES-IR) or
-    "gmres64" (GMRES on the FP64 matrix).
+    "gmres64" (GMRES on the FP64 matrix).  pivoting: "block" (pivots within
+    each 64 x 64 diagonal block, FP64 refinement, automatic fallback) or
+    "partial" (whole-column partial pivoting; with "lu64" the plain FP64 LU).
     """
 
-    def __init__(self, cluster, kind="lu64", tol=1e-16, maxit=300, restart=40, context=None):
+    def __init__(self, cluster, kind="lu64", tol=1e-16, maxit=300, restart=40, context=None, pivoting="block"):
         self.ctx = context or Context()
         self.kind = kind
         _check(self.ctx.L.omx_set_solver(self.ctx.h, KINDS[kind], tol, maxit, restart))
+        self.ctx.set_pivoting(pivoting)
         self.set_cluster(cluster)
 
     def set_cluster(self, c):

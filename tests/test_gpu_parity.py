@@ -36,9 +36,9 @@ def check_internal(ora, c, out, ref, tol):
     assert rel(out["x_int"], ref["x_int"]) < 100 * tol
 
 
-def make(ctx, c, kind="lu64"):
+def make(ctx, c, kind="lu64", pivoting="block"):
     from optimet_amd import solver
-    s = solver.Solver(c, kind=kind, context=ctx)
+    s = solver.Solver(c, kind=kind, context=ctx, pivoting=pivoting)
     s.update()
     return s
 
@@ -150,3 +150,47 @@ def test_medium_cluster_multi_panel(ctx, ora):
     s = make(ctx, c, "lu64")
     out = s.solve()
     assert rel(out["x_sca"], ref["x_sca"]) < FP64_TOL
+
+
+@pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL)])
+def test_partial_pivoting_mode(ctx, ora, kind, tol):
+    """Whole-column (tournament) pivoting: the same answers as block pivoting."""
+    c = small_drude(P=6, nmax=5, seed=5)
+    ref = ora.solve_all(c)
+    s = make(ctx, c, kind, pivoting="partial")
+    out = s.solve()
+    cs = s.cross_sections()
+    assert rel(out["x_sca"], ref["x_sca"]) < tol
+    assert rel(out["x_sca_sh"], ref["x_sca_sh"]) < 100 * tol
+    assert abs(cs["cext"] - ref["cext"]) / abs(ref["cext"]) < tol
+    ctx.set_pivoting("block")
+
+
+@pytest.mark.parametrize("kind", ["lu64", "lu32ir"])
+def test_block_pivoting_falls_back(ctx, kind):
+    """Zero diagonal blocks: block pivoting meets a zero pivot, the solver
+    refactors with partial pivoting and still returns the solution."""
+    rng = np.random.default_rng(8)
+    N = 256
+    A = np.zeros((N, N), complex)
+    A[:128, 128:] = rng.standard_normal((128, 128)) + 1j * rng.standard_normal((128, 128)) + 12 * np.eye(128)
+    A[128:, :128] = rng.standard_normal((128, 128)) + 1j * rng.standard_normal((128, 128)) + 12 * np.eye(128)
+    b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    before = ctx.stats()["pivot_fallbacks"]
+    x = ctx.dense_solve(A, b, kind)
+    assert ctx.stats()["pivot_fallbacks"] == before + 1
+    assert rel(x, np.linalg.solve(A, b)) < 1e-10
+
+
+@pytest.mark.parametrize("kind", ["lu64", "lu32ir"])
+def test_dense_solve_partial_mode(ctx, kind):
+    rng = np.random.default_rng(12)
+    N = 500
+    A = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    ctx.set_pivoting("partial")
+    try:
+        x = ctx.dense_solve(A, b, kind)
+    finally:
+        ctx.set_pivoting("block")
+    assert rel(x, np.linalg.solve(A, b)) < 1e-9
