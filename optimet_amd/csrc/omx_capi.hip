@@ -95,7 +95,7 @@ struct omx_context {
       d_coef, d_S64, d_S32, d_V64, d_V32, d_x, d_xsh, d_xint, d_xintsh, d_K, d_K1, d_cross, d_cg;
   DBuf d_piv, d_perm, d_cand, d_cnt, d_map, d_status, d_flags, d_wlu;
   hipStream_t aux = nullptr;
-  hipEvent_t lu_ev[2] = {nullptr, nullptr};
+  hipEvent_t lu_ev[4] = {nullptr, nullptr, nullptr, nullptr};
   DBuf d_basis, d_w1, d_w2, d_r, d_part, d_h, d_nrm, d_gpart;
   omx::LuWork lu{}, lu_sh{};
   bool profile = false;
@@ -144,7 +144,7 @@ void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags
   if(!second) {
     c->d_cand.ensure(sizeof(int) * 2 * gmax * omx::kNB, &c->total);
     c->d_cnt.ensure(sizeof(int) * 2 * gmax, &c->total);
-    c->d_map.ensure(sizeof(int) * (4 * omx::kNB + 4), &c->total);
+    c->d_map.ensure(sizeof(int) * (3 * ((size_t)Np + 2 * omx::kNB)), &c->total);
     c->d_status.ensure(64, &c->total);
     c->d_wlu.ensure(sizeof(cplx) * 3 * omx::kNB * omx::kNB, &c->total);
   }
@@ -159,8 +159,8 @@ void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags
   w.cnt[0] = c->d_cnt.as<int>();
   w.cnt[1] = c->d_cnt.as<int>() + gmax;
   w.mdst = c->d_map.as<int>();
-  w.msrc = c->d_map.as<int>() + 2 * omx::kNB;
-  w.mcnt = c->d_map.as<int>() + 4 * omx::kNB;
+  w.msrc = c->d_map.as<int>() + (Np + 2 * omx::kNB);
+  w.mcnt = c->d_map.as<int>() + 2 * (Np + 2 * omx::kNB);
   w.status = c->d_status.as<int>();
   w.wlu = c->d_wlu.p;
   w.dinv = dinv.p;
@@ -168,6 +168,8 @@ void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags
   w.aux = c->aux;
   w.ev_next = c->lu_ev[0];
   w.ev_piv = c->lu_ev[1];
+  w.ev_bulk = c->lu_ev[2];
+  w.ev_pan = c->lu_ev[3];
   w.gmax = gmax;
   w.epoch = 0;
 }

@@ -90,14 +90,14 @@ struct LuWork {
   int *perm;            // Np: final row permutation (row i of P A is row perm[i] of A)
   int *cand[2];         // tournament candidates, ping-pong, [Gmax][NB]
   int *cnt[2];          // candidates per set, [Gmax]
-  int *mdst, *msrc;     // gather map of one panel, 2 NB entries
-  int *mcnt;
+  int *mdst, *msrc;     // gather maps: tournament 2 NB entries at the start; block
+  int *mcnt;            // pivoting NB entries per panel (panel k at k NB, count mcnt[k])
   int *status;          // [0]: singular pivot column+1, [1]: wait timeout
   int *flags;           // Np / NB triangular-solve flags
   void *wlu;            // NB x NB complex: the tournament's selected-block factors
   void *dinv;           // per panel [L11^{-1} | U11^{-1}], 2 NB^2 elements of T, column-major
   hipStream_t aux;      // look-ahead stream of the panel pivoting
-  hipEvent_t ev_next, ev_piv;
+  hipEvent_t ev_next, ev_piv, ev_bulk, ev_pan;
   int gmax;
   int epoch;            // last flag value used by a triangular solve
   int pivot;            // kPivotBlock or kPivotPartial

@@ -551,37 +551,21 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   }
 }
 
-// In-place block product with a 64 x 64 factor M (column-major, ld 64):
-// LEFT: the 64 x 64 block at (j0, c0) becomes M * block (U12 = L11^{-1} A12);
-// RIGHT: the block at (r0, j0) becomes block * M (L21 = A21 U11^{-1}).
+// In-place block product with a 64 x 64 factor M (column-major):
+//   LEFT:  A[j0 + i, off + n] = sum_k M(i, k) A[j0 + k, off + n]
+//   RIGHT: A[off + m, j0 + n] = sum_k A[off + m, j0 + k] M(k, n)
+// with off = first + 64 blockIdx.x.  K is staged through LDS in chunks of
+// 16 (a small LDS footprint lets these workgroups start beside the bulk
+// update's); the block is overwritten only after all of it has been read.
 template <typename T, bool LEFT>
 __global__ __launch_bounds__(256) void blk_mul(T *A, size_t lda, int j0, int first, const T *M) {
   using R = typename CT<T>::R;
   using V4 = typename CT<T>::V4;
-  constexpr int PAD = 4;
-  __shared__ R As[2][64][64 + PAD];  // [plane][k][m]
-  __shared__ R Bs[2][64][64 + PAD];  // [plane][k][n]
+  constexpr int BK = 16, PAD = 16;
+  __shared__ R As[2][BK][64 + PAD];  // [plane][k][m]
+  __shared__ R Bs[2][BK][64 + PAD];  // [plane][k][n]
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, wm = wv & 1, wn = wv >> 1;
-  const int off = first + blockIdx.x * 64;  // column (LEFT) or row (RIGHT) origin of the block
-  for(int idx = t; idx < 64 * 64; idx += 256) {
-    const int i = idx & 63, j = idx >> 6;  // i fastest
-    if(LEFT) {
-      T m = M[j * 64 + i];                              // M(i, k=j)
-      As[0][j][i] = m.x;
-      As[1][j][i] = m.y;
-      T b = A[(size_t)(off + j) * lda + j0 + i];        // block(k=i, n=j)
-      Bs[0][i][j] = b.x;
-      Bs[1][i][j] = b.y;
-    } else {
-      T a = A[(size_t)(j0 + j) * lda + off + i];        // block(m=i, k=j)
-      As[0][j][i] = a.x;
-      As[1][j][i] = a.y;
-      T m = M[j * 64 + i];                              // M(k=i, n=j)
-      Bs[0][i][j] = m.x;
-      Bs[1][i][j] = m.y;
-    }
-  }
-  __syncthreads();
+  const int off = first + blockIdx.x * 64;
   V4 cr[2][2], ci[2][2];
 #pragma unroll
   for(int a = 0; a < 2; ++a)
@@ -590,28 +574,52 @@ __global__ __launch_bounds__(256) void blk_mul(T *A, size_t lda, int j0, int fir
       cr[a][b] = V4{0, 0, 0, 0};
       ci[a][b] = V4{0, 0, 0, 0};
     }
-#pragma unroll 4
-  for(int ks = 0; ks < 16; ++ks) {
-    const int kr = ks * 4 + (lane >> 4), li = lane & 15;
-    R ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll 1
+  for(int kc = 0; kc < 64; kc += BK) {
+    __syncthreads();
 #pragma unroll
-    for(int a = 0; a < 2; ++a) {
-      ar[a] = As[0][kr][wm * 32 + a * 16 + li];
-      ai[a] = As[1][kr][wm * 32 + a * 16 + li];
-      nai[a] = -ai[a];
-      br[a] = Bs[0][kr][wn * 32 + a * 16 + li];
-      bi[a] = Bs[1][kr][wn * 32 + a * 16 + li];
-    }
-#pragma unroll
-    for(int a = 0; a < 2; ++a)
-#pragma unroll
-      for(int b = 0; b < 2; ++b) {
-        cr[a][b] = CT<T>::mfma(ar[a], br[b], cr[a][b]);
-        cr[a][b] = CT<T>::mfma(nai[a], bi[b], cr[a][b]);
-        ci[a][b] = CT<T>::mfma(ar[a], bi[b], ci[a][b]);
-        ci[a][b] = CT<T>::mfma(ai[a], br[b], ci[a][b]);
+    for(int it = 0; it < 4; ++it) {
+      const int idx = t + 256 * it;
+      // left operand (m = idx % 64, k = idx / 64), right operand (k = idx % 16, n = idx / 16)
+      const int m = idx & 63, ka = idx >> 6, kb = idx & 15, n = idx >> 4;
+      T av, bv;
+      if(LEFT) {
+        av = M[(kc + ka) * 64 + m];                              // M(m, k)
+        bv = A[(size_t)(off + n) * lda + j0 + kc + kb];          // block(k, n)
+      } else {
+        av = A[(size_t)(j0 + kc + ka) * lda + off + m];          // block(m, k)
+        bv = M[n * 64 + kc + kb];                                // M(k, n)
       }
+      As[0][ka][m] = av.x;
+      As[1][ka][m] = av.y;
+      Bs[0][kb][n] = bv.x;
+      Bs[1][kb][n] = bv.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for(int ks = 0; ks < BK / 4; ++ks) {
+      const int kr = ks * 4 + (lane >> 4), li = lane & 15;
+      R ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+      for(int a = 0; a < 2; ++a) {
+        ar[a] = As[0][kr][wm * 32 + a * 16 + li];
+        ai[a] = As[1][kr][wm * 32 + a * 16 + li];
+        nai[a] = -ai[a];
+        br[a] = Bs[0][kr][wn * 32 + a * 16 + li];
+        bi[a] = Bs[1][kr][wn * 32 + a * 16 + li];
+      }
+#pragma unroll
+      for(int a = 0; a < 2; ++a)
+#pragma unroll
+        for(int b = 0; b < 2; ++b) {
+          cr[a][b] = CT<T>::mfma(ar[a], br[b], cr[a][b]);
+          ci[a][b] = CT<T>::mfma(ar[a], bi[b], ci[a][b]);
+          cr[a][b] = CT<T>::mfma(nai[a], bi[b], cr[a][b]);
+          ci[a][b] = CT<T>::mfma(ai[a], br[b], ci[a][b]);
+        }
+    }
   }
+  __syncthreads();  // every workgroup thread has read the block before it is overwritten
 #pragma unroll
   for(int a = 0; a < 2; ++a)
 #pragma unroll
@@ -632,20 +640,21 @@ __global__ __launch_bounds__(256) void blk_mul(T *A, size_t lda, int j0, int fir
 }
 
 // ------------------------------------------------------------------ swaps
-// Applies the panel's gather map to every column: each workgroup handles 8
-// columns, reads all moved entries into registers, then writes them back.
-// Columns skip0 .. skip0 + NB - 1 are left out when skip0 >= 0.
+// Applies a gather map (rows mdst[i] <- msrc[i], i < *mcnt <= 128) to the
+// columns c0 .. cend-1, leaving out skip0 .. skip0 + NB - 1 when skip0 >= 0:
+// each workgroup handles 8 columns, reads all moved entries into registers,
+// then writes them back.
 template <typename T>
-__global__ __launch_bounds__(256) void laswp_map(T *A, size_t lda, int Np, const int *__restrict__ mdst,
+__global__ __launch_bounds__(256) void laswp_map(T *A, size_t lda, int c0, int cend, const int *__restrict__ mdst,
                                                  const int *__restrict__ msrc, const int *__restrict__ mcnt,
                                                  int skip0) {
   const int m = *mcnt;
   const int cl = threadIdx.x / 32, li = threadIdx.x % 32;
-  int col = blockIdx.x * 8 + cl;
+  int col = c0 + blockIdx.x * 8 + cl;
   if(skip0 >= 0 && col >= skip0) col += kNB;
   T v[4];
   int d[4];
-  if(col < Np) {
+  if(col < cend) {
     T *a = A + (size_t)col * lda;
 #pragma unroll
     for(int k = 0; k < 4; ++k) {
@@ -655,11 +664,48 @@ __global__ __launch_bounds__(256) void laswp_map(T *A, size_t lda, int Np, const
     }
   }
   __syncthreads();
-  if(col < Np) {
+  if(col < cend) {
     T *a = A + (size_t)col * lda;
 #pragma unroll
     for(int k = 0; k < 4; ++k)
       if(d[k] >= 0) a[d[k]] = v[k];
+  }
+}
+template <typename T>
+void laswp(T *A, size_t lda, int c0, int cend, const int *mdst, const int *msrc, const int *mcnt, int skip0,
+           hipStream_t st) {
+  const int n = cend - c0 - (skip0 >= 0 ? kNB : 0);
+  if(n > 0) hipLaunchKernelGGL((laswp_map<T>), dim3((n + 7) / 8), dim3(256), 0, st, A, lda, c0, cend, mdst, msrc, mcnt, skip0);
+}
+
+// Deferred interchanges of block pivoting: panel k = blockIdx.y (>= 1)
+// applies its map (NB entries at k NB) to the columns left of it.
+template <typename T>
+__global__ __launch_bounds__(256) void laswp_left(T *A, size_t lda, const int *__restrict__ mdst,
+                                                  const int *__restrict__ msrc, const int *__restrict__ mcnt) {
+  const int k = blockIdx.y + 1;
+  const int cl = threadIdx.x / 32, li = threadIdx.x % 32;
+  const int col = blockIdx.x * 8 + cl;
+  if(blockIdx.x * 8 >= k * kNB) return;  // whole workgroup past the panel
+  const int m = mcnt[k];
+  const int *md = mdst + k * kNB, *ms = msrc + k * kNB;
+  T v[2];
+  int d[2];
+  if(col < k * kNB) {
+    const T *a = A + (size_t)col * lda;
+#pragma unroll
+    for(int q = 0; q < 2; ++q) {
+      const int i = li + 32 * q;
+      d[q] = i < m ? md[i] : -1;
+      if(i < m) v[q] = a[ms[i]];
+    }
+  }
+  __syncthreads();
+  if(col < k * kNB) {
+    T *a = A + (size_t)col * lda;
+#pragma unroll
+    for(int q = 0; q < 2; ++q)
+      if(d[q] >= 0) a[d[q]] = v[q];
   }
 }
 
@@ -788,9 +834,10 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_tile(T *A, size_t lda, in
     }
 }
 
-// Tile configuration of the trailing updates.
+// Tile configuration of the trailing updates (fastest of a sweep at
+// M = N = 10112, K = 128: 103 TFLOP/s complex FP32, 55 TFLOP/s complex FP64).
 template <typename T> struct UpdTile {
-  static constexpr int TM = 64, TN = 64, WGM = 2, WGN = 2, BK = 16;
+  static constexpr int TM = 128, TN = 64, WGM = 4, WGN = 2, BK = 16;
 };
 
 // C -= A B over an M x N block (M, N multiples of 64) with K = KD.
@@ -804,17 +851,22 @@ void trailing_update(T *A, size_t lda, int row0, int col0, int k0, int M, int N,
 }
 
 // ------------------------------------------------------------------ driver
-// Panels of NB columns, taken in pairs (p0, p1) so that the bulk of the
-// trailing update runs with K = 2 NB.  Per pair, on the main stream st:
-//   panel p0      wait for its pivoting (issued earlier on the look-ahead
-//                 stream sb), laswp_map, [panel_inv], blk_mul L21 and U12
-//   update        p1's columns with p0 (K = NB); then block row p1 beyond
-//                 p1 with p0 (K = NB): with block pivoting while sb pivots
-//                 p1, with the tournament after p1's interchanges
-//   panel p1      as p0
-//   bulk update   A22 -= [L21(p0) L21(p1)] [U12(p0); U12(p1)] (K = 2 NB): the
-//                 next pair's first columns first, then, while sb pivots that
-//                 panel, the rest
+// Panels of NB columns are taken in pairs (p0 = J, p1 = J + NB) so that the
+// bulk of the trailing update runs with K = 2 NB.
+//
+// Block pivoting (interchanges stay inside a panel's 64 rows), two streams:
+// sb (panel work) runs pair s+1 while st (bulk updates) finishes pair s.
+//   sb  diag_lu p0; swaps on p1's columns; L21(p0); U12(p0) on p1's columns;
+//       p1's columns -= L21(p0) U12(p0); diag_lu p1; L21(p1)
+//   sb  after st's bulk update of pair s-1: swaps of p0 on the far columns,
+//       U12(p0), block row p1 -= L(p1, p0) U12(p0), swaps of p1, U12(p1)
+//   st  A22 -= [L21(p0) L21(p1)] [U12(p0); U12(p1)]: the next pair's
+//       columns first (sb starts on them), then the rest
+// Interchanges on the columns left of each panel are applied once at the end
+// (panels touch disjoint rows, so their order does not matter).
+//
+// Tournament (partial) pivoting: one stream for the panel sequence, sb for
+// the tournaments; every interchange is applied at once to all columns.
 template <typename T>
 int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof) {
   if(Np % kNB) return -1;
@@ -822,89 +874,131 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
   hipStream_t sb = w.aux;
   T *wlu = reinterpret_cast<T *>(w.wlu);
   T *dinv = reinterpret_cast<T *>(w.dinv);
+  auto dk = [&](int j0) { return dinv + (size_t)(j0 / kNB) * 2 * kNB * kNB; };
   hipLaunchKernelGGL(iota_kernel, dim3((Np + 255) / 256), dim3(256), 0, st, w.perm, Np);
   if(prof) {
     prof->used = 0;
     prof->gemm_flop = 0.0;
   }
+  if(block) {
+    auto map = [&](int j0, int **d, int **s_, int **c) {
+      const int k = j0 / kNB;
+      *d = w.mdst + k * kNB;
+      *s_ = w.msrc + k * kNB;
+      *c = w.mcnt + k;
+    };
+    (void)hipEventRecord(w.ev_next, st);
+    (void)hipEventRecord(w.ev_bulk, st);
+    for(int J = 0, pi = 0; J < Np; J += 2 * kNB, ++pi) {
+      const int p0 = J, p1 = J + kNB, far = J + 2 * kNB;
+      const bool two = p1 < Np;
+      const int rest0 = Np - p0 - kNB, restf = Np - far;
+      const bool rec = prof && pi < prof->cap;
+      hipEvent_t *pe = rec ? prof->ev + 4 * pi : nullptr;
+      int *d0, *s0, *c0, *d1, *s1, *c1;
+      map(p0, &d0, &s0, &c0);
+      map(p1, &d1, &s1, &c1);
+      (void)hipStreamWaitEvent(sb, w.ev_next, 0);
+      if(rec) (void)hipEventRecord(pe[0], sb);
+      hipLaunchKernelGGL((diag_lu<T>), dim3(1), dim3(512), 0, sb, A, lda, p0, dk(p0), d0, s0, c0, w.perm, w.status);
+      if(rest0 > 0)
+        hipLaunchKernelGGL((blk_mul<T, false>), dim3(rest0 / 64), dim3(256), 0, sb, A, lda, p0, p1, dk(p0) + kNB * kNB);
+      if(two) {
+        laswp<T>(A, lda, p1, p1 + kNB, d0, s0, c0, -1, sb);
+        hipLaunchKernelGGL((blk_mul<T, true>), dim3(1), dim3(256), 0, sb, A, lda, p0, p1, dk(p0));
+        trailing_update<T, kNB>(A, lda, p1, p1, p0, Np - p1, kNB, sb);
+        hipLaunchKernelGGL((diag_lu<T>), dim3(1), dim3(512), 0, sb, A, lda, p1, dk(p1), d1, s1, c1, w.perm, w.status);
+        if(restf > 0)
+          hipLaunchKernelGGL((blk_mul<T, false>), dim3(restf / 64), dim3(256), 0, sb, A, lda, p1, far,
+                             dk(p1) + kNB * kNB);
+      }
+      if(rec) (void)hipEventRecord(pe[1], sb);
+      if(two && restf > 0) {
+        (void)hipStreamWaitEvent(sb, w.ev_bulk, 0);
+        laswp<T>(A, lda, far, Np, d0, s0, c0, -1, sb);
+        hipLaunchKernelGGL((blk_mul<T, true>), dim3(restf / 64), dim3(256), 0, sb, A, lda, p0, far, dk(p0));
+        trailing_update<T, kNB>(A, lda, p1, far, p0, kNB, restf, sb);
+        laswp<T>(A, lda, far, Np, d1, s1, c1, -1, sb);
+        hipLaunchKernelGGL((blk_mul<T, true>), dim3(restf / 64), dim3(256), 0, sb, A, lda, p1, far, dk(p1));
+      }
+      (void)hipEventRecord(w.ev_pan, sb);
+      (void)hipStreamWaitEvent(st, w.ev_pan, 0);
+      if(rec) (void)hipEventRecord(pe[2], st);
+      if(two && restf > 0) {
+        const int nn = restf < 2 * kNB ? restf : 2 * kNB;
+        trailing_update<T, 2 * kNB>(A, lda, far, far, p0, restf, nn, st);
+        (void)hipEventRecord(w.ev_next, st);
+        trailing_update<T, 2 * kNB>(A, lda, far, far + nn, p0, restf, restf - nn, st);
+        (void)hipEventRecord(w.ev_bulk, st);
+        if(rec) prof->gemm_flop += 8.0 * (double)restf * restf * (2 * kNB);
+      }
+      if(rec) {
+        (void)hipEventRecord(pe[3], st);
+        prof->used = pi + 1;
+      }
+    }
+    if(Np > kNB)
+      hipLaunchKernelGGL((laswp_left<T>), dim3((Np - kNB) / 8, Np / kNB - 1), dim3(256), 0, st, A, lda, w.mdst, w.msrc,
+                         w.mcnt);
+    return 0;
+  }
+
+  // ---- tournament pivoting
   auto pivoting = [&](int j0) {
     (void)hipEventRecord(w.ev_next, st);
     (void)hipStreamWaitEvent(sb, w.ev_next, 0);
-    if(block) {
-      hipLaunchKernelGGL((diag_lu<T>), dim3(1), dim3(512), 0, sb, A, lda, j0, dinv + (size_t)(j0 / kNB) * 2 * kNB * kNB,
-                         w.mdst, w.msrc, w.mcnt, w.perm, w.status);
-    } else {
-      const int G = (Np - j0 + kPanelRows - 1) / kPanelRows;
-      hipLaunchKernelGGL((tslu_leaf<T>), dim3(G), dim3(256), 0, sb, A, lda, Np, j0, w.cand[0], w.cnt[0], wlu);
-      int gin = G, buf = 0;
-      while(gin > 1) {
-        const int gout = (gin + 1) / 2;
-        hipLaunchKernelGGL((tslu_merge<T>), dim3(gout), dim3(256), 0, sb, A, lda, j0, gin, w.cand[buf], w.cnt[buf],
-                           w.cand[buf ^ 1], w.cnt[buf ^ 1], wlu);
-        buf ^= 1;
-        gin = gout;
-      }
-      hipLaunchKernelGGL(tslu_finish, dim3(1), dim3(64), 0, sb, w.cand[buf], w.cnt[buf], j0, w.piv, w.mdst, w.msrc,
-                         w.mcnt, w.perm);
+    const int G = (Np - j0 + kPanelRows - 1) / kPanelRows;
+    hipLaunchKernelGGL((tslu_leaf<T>), dim3(G), dim3(256), 0, sb, A, lda, Np, j0, w.cand[0], w.cnt[0], wlu);
+    int gin = G, buf = 0;
+    while(gin > 1) {
+      const int gout = (gin + 1) / 2;
+      hipLaunchKernelGGL((tslu_merge<T>), dim3(gout), dim3(256), 0, sb, A, lda, j0, gin, w.cand[buf], w.cnt[buf],
+                         w.cand[buf ^ 1], w.cnt[buf ^ 1], wlu);
+      buf ^= 1;
+      gin = gout;
     }
+    hipLaunchKernelGGL(tslu_finish, dim3(1), dim3(64), 0, sb, w.cand[buf], w.cnt[buf], j0, w.piv, w.mdst, w.msrc,
+                       w.mcnt, w.perm);
     (void)hipEventRecord(w.ev_piv, sb);
   };
-  // the pivoting's row interchanges and (tournament) the block's factors
+  // interchanges on every column and the block's factors and inverses
   auto swaps = [&](int j0) {
-    T *dk = dinv + (size_t)(j0 / kNB) * 2 * kNB * kNB;
     (void)hipStreamWaitEvent(st, w.ev_piv, 0);
-    if(block) {
-      // the diagonal block was factored in pivoted order by diag_lu
-      if(Np > kNB)
-        hipLaunchKernelGGL((laswp_map<T>), dim3((Np - kNB + 7) / 8), dim3(256), 0, st, A, lda, Np, w.mdst, w.msrc,
-                           w.mcnt, j0);
-    } else {
-      hipLaunchKernelGGL((laswp_map<T>), dim3((Np + 7) / 8), dim3(256), 0, st, A, lda, Np, w.mdst, w.msrc, w.mcnt,
-                         -1);
-      hipLaunchKernelGGL((panel_inv<T>), dim3(1), dim3(256), 0, st, A, lda, j0, wlu, dk, w.status);
-    }
+    laswp<T>(A, lda, 0, Np, w.mdst, w.msrc, w.mcnt, -1, st);
+    hipLaunchKernelGGL((panel_inv<T>), dim3(1), dim3(256), 0, st, A, lda, j0, wlu, dk(j0), w.status);
   };
-  // L21 and U12 of panel j0 (after its swaps)
+  // L21 and U12 of panel j0
   auto solves = [&](int j0) {
     const int rest = Np - j0 - kNB;
-    T *dk = dinv + (size_t)(j0 / kNB) * 2 * kNB * kNB;
     if(rest > 0) {
-      hipLaunchKernelGGL((blk_mul<T, true>), dim3(rest / 64), dim3(256), 0, st, A, lda, j0, j0 + kNB, dk);
+      hipLaunchKernelGGL((blk_mul<T, true>), dim3(rest / 64), dim3(256), 0, st, A, lda, j0, j0 + kNB, dk(j0));
       hipLaunchKernelGGL((blk_mul<T, false>), dim3(rest / 64), dim3(256), 0, st, A, lda, j0, j0 + kNB,
-                         dk + kNB * kNB);
+                         dk(j0) + kNB * kNB);
     }
   };
-  auto panel = [&](int j0) {
-    swaps(j0);
-    solves(j0);
-  };
-  if(!block && (Np + kPanelRows - 1) / kPanelRows > w.gmax) return -2;
+  if((Np + kPanelRows - 1) / kPanelRows > w.gmax) return -2;
   pivoting(0);
   for(int J = 0, pi = 0; J < Np; J += 2 * kNB, ++pi) {
     const int p0 = J, p1 = J + kNB;
     const bool rec = prof && pi < prof->cap;
     hipEvent_t *pe = rec ? prof->ev + 4 * pi : nullptr;
     if(rec) (void)hipEventRecord(pe[0], st);
-    panel(p0);
-    const int rest1 = Np - p1 - kNB;  // beyond the pair
+    swaps(p0);
+    solves(p0);
+    const int rest1 = Np - p1 - kNB;
     if(p1 < Np) {
       trailing_update<T, kNB>(A, lda, p1, p1, p0, Np - p1, kNB, st);
       pivoting(p1);
-      if(block) {
-        // interchanges stay inside block row p1, so its update with p0 can
-        // run while p1 is pivoted
-        trailing_update<T, kNB>(A, lda, p1, p1 + kNB, p0, kNB, rest1, st);
-        panel(p1);
-      } else {
-        // rows from below (not yet updated with p0 beyond p1) may move into
-        // block row p1: interchange first, then update the block row
-        swaps(p1);
-        trailing_update<T, kNB>(A, lda, p1, p1 + kNB, p0, kNB, rest1, st);
-        solves(p1);
-      }
+      // rows from below (not yet updated with p0 beyond p1) may move into
+      // block row p1: interchange first, then update the block row
+      swaps(p1);
+      trailing_update<T, kNB>(A, lda, p1, p1 + kNB, p0, kNB, rest1, st);
+      solves(p1);
     }
-    if(rec) (void)hipEventRecord(pe[1], st);
-    if(rec) (void)hipEventRecord(pe[2], st);
+    if(rec) {
+      (void)hipEventRecord(pe[1], st);
+      (void)hipEventRecord(pe[2], st);
+    }
     if(p1 < Np && rest1 > 0) {
       const int Jn = p1 + kNB;
       trailing_update<T, 2 * kNB>(A, lda, Jn, Jn, p0, rest1, kNB, st);
