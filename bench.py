@@ -30,6 +30,37 @@ sys.path.insert(0, ROOT)
 FP32_MFMA_TFLOPS = 157.3   # MI355X dense FP32 matrix peak
 FP64_MFMA_TFLOPS = 78.6    # MI355X dense FP64 matrix peak
 HBM_GBS = 8000.0
+GEMM_KERNEL = {"lu32ir": "gemm_tile<float2,128,64,128,4,2,16>", "lu64": "gemm_tile<double2,128,64,128,4,2,16>"}
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "traffic_gemm_tile_c64_config2.txt")
+
+
+def traffic_per_launch():
+    """HBM bytes per launch of the FP32 bulk update from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    try:
+        vals = {}
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if "=" in line and not line.startswith("#"):
+                    k, v = line.split("=", 1)
+                    vals[k.strip()] = float(v.split()[0])
+        return vals["fetch_bytes_per_launch"] + vals["write_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def roofline(stats, kind, peak):
+    """Bulk trailing update of the LU: algorithmic flop (8 m n k per launch)
+    over the HIP-event time of its launches on the stream they run on."""
+    t = stats["t_lu_gemm_ms"] * 1e-3
+    n = stats["lu_gemm_launches"]
+    if t <= 0 or n <= 0:
+        return None
+    achieved = stats["lu_gemm_flop"] / t / 1e12
+    return {"bound": "mfma", "kernel": GEMM_KERNEL[kind], "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "launches": n, "avg_launch_us": 1e6 * t / n,
+            "flop_per_launch": stats["lu_gemm_flop"] / n}
 
 
 def pct(v, q):
@@ -68,7 +99,7 @@ def cpu_baseline(cluster):
     O(N^3) (the reference's ColPivHouseholderQR), SH source O(P)."""
     import oracle
 
-    npairs, nsolve = 512, 1536
+    npairs, nsolve = 1024, 2048
     t_pair, t_solve, t_sh = oracle.time_sample(cluster, npairs, nsolve, 1)
     P, N = cluster.nobj, cluster.N
     est = t_pair * (2 * P * (P - 1) + 2 * P) + 2 * t_solve * (N / nsolve) ** 3 + t_sh * P
@@ -85,7 +116,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 LU comparison path")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized device buffer (calibration of rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this single-GPU build "
               "(the multi-GPU layout is described in DESIGN.md)", file=sys.stderr)
@@ -99,7 +143,9 @@ def main():
 
     ms = mixed["stats"]
     value = args.steps / mixed["total"]
-    gemm_tf = ms["lu_gemm_flop"] / (ms["t_lu_gemm_ms"] * 1e-3) / 1e12 if ms["t_lu_gemm_ms"] > 0 else None
+    roof = roofline(ms, "lu32ir", FP32_MFMA_TFLOPS)
+    if roof is not None:
+        roof["traffic"] = traffic_per_launch()
     line = {
         "metric": "cluster solves/sec + p99 latency, mixed-precision vs FP64 LU at rel err <=1e-9",
         "value": value,
@@ -122,17 +168,15 @@ def main():
         "residual": [ms["residual"], ms["residual_sh"]],
         "lu_factor_ms_total": ms["t_factor_total_ms"],
         "lu_panel_ms_total": ms["t_lu_panel_ms"],
-        "roofline": {"bound": "mfma", "kernel": "gemm_kernel<float2> (LU trailing update)",
-                     "achieved": gemm_tf, "peak": FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": (gemm_tf / FP32_MFMA_TFLOPS) if gemm_tf else None, "traffic": None},
+        "roofline": roof,
     }
     if fp64 is not None:
         fs = fp64["stats"]
-        g64 = fs["lu_gemm_flop"] / (fs["t_lu_gemm_ms"] * 1e-3) / 1e12 if fs["t_lu_gemm_ms"] > 0 else None
         rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
         line["fp64_lu"] = {"solves_per_s": args.steps / fp64["total"], "p50_ms": 1e3 * pct(fp64["lat"], 50),
-                           "p99_ms": 1e3 * pct(fp64["lat"], 99), "lu_gemm_tflops": g64,
-                           "lu_gemm_frac_fp64_peak": (g64 / FP64_MFMA_TFLOPS) if g64 else None,
+                           "p99_ms": 1e3 * pct(fp64["lat"], 99),
+                           "roofline": roofline(fs, "lu64", FP64_MFMA_TFLOPS),
+                           "iterations": [fs["iterations"], fs["iterations_sh"]],
                            "lu_factor_ms_total": fs["t_factor_total_ms"], "lu_panel_ms_total": fs["t_lu_panel_ms"]}
         line["speedup_vs_fp64"] = value / line["fp64_lu"]["solves_per_s"]
         mo, fo = mixed["out"], fp64["out"]

@@ -73,12 +73,13 @@ typedef struct {
   double t_sh_source_ms; /* last omx_solve: second-harmonic source vector */
   double t_assemble_ms;  /* last omx_update: pair-coefficient kernels only */
   size_t device_bytes;   /* device memory held by the context */
-  /* with omx_set_profile(ctx, 1): per-kernel time on the context's stream */
-  double t_lu_gemm_ms;   /* trailing-update GEMM launches of the last factorisations */
+  /* with omx_set_profile(ctx, 1), accumulated over the factorisations since:
+   * HIP-event times on the streams the kernels run on */
+  double t_lu_gemm_ms;   /* bulk trailing updates (K = 128) */
   double lu_gemm_flop;   /* their algorithmic flop, 8 m n k per launch */
-  int lu_gemm_launches;
-  double t_lu_panel_ms;  /* panel factorisation launches */
-  double t_factor_total_ms; /* whole factorisations (panel, swaps, TRSM, GEMM) */
+  int lu_gemm_launches;  /* their kernel launches */
+  double t_lu_panel_ms;  /* panel work of each pair up to its far-column step */
+  double t_factor_total_ms; /* whole factorisations */
   int pivot_fallbacks;   /* refactorisations with partial pivoting since creation */
 } omx_stats;
 

@@ -388,11 +388,11 @@ void prof_end(omx_context *c) {
     c->prof_panel_ms += ms;
     HIPCHK(hipEventElapsedTime(&ms, e[2], e[3]));
     c->prof_gemm_ms += ms;
-    if(ms > 0) ++c->prof_launches;
   }
   HIPCHK(hipEventElapsedTime(&ms, c->pev[0], c->pev[1]));
   c->prof_total_ms += ms;
   c->prof_flop += c->prof.gemm_flop;
+  c->prof_launches += c->prof.gemm_launches;
 }
 
 // Solves (S or V) x = rhs with the context's solver kind.

@@ -109,10 +109,12 @@ constexpr int kPivotPartial = 1;
 
 // Optional per-launch timing of the factorisation (events on the same stream).
 struct LuProf {
-  hipEvent_t *ev = nullptr;  // 4 events per panel: panel start/end, gemm start/end
-  int cap = 0;               // panels the pool covers
-  int used = 0;              // panels recorded by the last factorisation
-  double gemm_flop = 0.0;    // algorithmic flop of the recorded trailing updates
+  hipEvent_t *ev = nullptr;  // 4 events per panel pair: panel work start/end (look-ahead
+                             // stream), bulk update start/end (main stream)
+  int cap = 0;               // pairs the pool covers
+  int used = 0;              // pairs recorded by the last factorisation
+  double gemm_flop = 0.0;    // algorithmic flop of the recorded bulk updates (8 m n k each)
+  int gemm_launches = 0;     // their kernel launches
 };
 
 // Factor the Np x Np complex matrix in place (P A = L U); T = double2 or float2.

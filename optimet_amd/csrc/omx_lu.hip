@@ -879,6 +879,7 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
   if(prof) {
     prof->used = 0;
     prof->gemm_flop = 0.0;
+    prof->gemm_launches = 0;
   }
   if(block) {
     auto map = [&](int j0, int **d, int **s_, int **c) {
@@ -930,7 +931,10 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
         (void)hipEventRecord(w.ev_next, st);
         trailing_update<T, 2 * kNB>(A, lda, far, far + nn, p0, restf, restf - nn, st);
         (void)hipEventRecord(w.ev_bulk, st);
-        if(rec) prof->gemm_flop += 8.0 * (double)restf * restf * (2 * kNB);
+        if(rec) {
+          prof->gemm_flop += 8.0 * (double)restf * restf * (2 * kNB);
+          prof->gemm_launches += restf > nn ? 2 : 1;
+        }
       }
       if(rec) {
         (void)hipEventRecord(pe[3], st);
@@ -1004,7 +1008,10 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
       trailing_update<T, 2 * kNB>(A, lda, Jn, Jn, p0, rest1, kNB, st);
       pivoting(Jn);
       trailing_update<T, 2 * kNB>(A, lda, Jn, Jn + kNB, p0, rest1, rest1 - kNB, st);
-      if(rec) prof->gemm_flop += 8.0 * (double)rest1 * rest1 * (2 * kNB);
+      if(rec) {
+        prof->gemm_flop += 8.0 * (double)rest1 * rest1 * (2 * kNB);
+        prof->gemm_launches += rest1 > kNB ? 2 : 1;
+      }
     }
     if(rec) {
       (void)hipEventRecord(pe[3], st);

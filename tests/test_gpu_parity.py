@@ -194,3 +194,22 @@ def test_dense_solve_partial_mode(ctx, kind):
     finally:
         ctx.set_pivoting("block")
     assert rel(x, np.linalg.solve(A, b)) < 1e-9
+
+
+@pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL)])
+def test_golden_fixtures_gpu(ctx, kind, tol):
+    """The HIP path against the committed fixtures (tests/golden/make_golden.py)."""
+    import os
+    from tests.golden.make_golden import CASES
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    for name, build in CASES.items():
+        ref = np.load(os.path.join(here, name + ".npz"))
+        s = make(ctx, build(), kind)
+        out = s.solve()
+        cs = s.cross_sections()
+        assert rel(out["x_sca"], ref["x_sca"]) < tol, name
+        for key in ("cext", "csca", "cabs"):
+            assert abs(cs[key] - ref[key]) <= tol * abs(ref[key]), (name, key)
+        if "x_sca_sh" in ref.files:
+            assert rel(out["x_sca_sh"], ref["x_sca_sh"]) < 100 * tol, name
+            assert abs(cs["csca_sh"] - ref["csca_sh"]) <= 100 * tol * abs(ref["csca_sh"]), name

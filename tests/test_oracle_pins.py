@@ -308,3 +308,96 @@ def test_cg_table_symmetries(ora):
     forbidden = m[:, None, None] != (m[None, :, None] + m[None, None, :])
     assert np.all(t[:, forbidden] == 0)
     assert np.max(np.abs(t)) > 0.01
+
+
+def _rotate_problem(c, R):
+    """The same cluster and incident wave rotated rigidly by R (3x3)."""
+    th, ph = c.theta_k, c.phi_k
+    k = np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+    et = np.array([np.cos(th) * np.cos(ph), np.cos(th) * np.sin(ph), -np.sin(th)])
+    ep = np.array([-np.sin(ph), np.cos(ph), 0.0])
+    E = c.etheta * et + c.ephi * ep
+    k2, E2 = R @ k, R @ E
+    th2, ph2 = np.arccos(np.clip(k2[2], -1, 1)), np.arctan2(k2[1], k2[0])
+    et2 = np.array([np.cos(th2) * np.cos(ph2), np.cos(th2) * np.sin(ph2), -np.sin(th2)])
+    ep2 = np.array([-np.sin(ph2), np.cos(ph2), 0.0])
+    import dataclasses
+    return dataclasses.replace(c, xyz=(R @ c.xyz.T).T, theta_k=th2, phi_k=ph2, etheta=complex(E2 @ et2),
+                               ephi=complex(E2 @ ep2))
+
+
+def _rot(axis, ang):
+    a = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+
+
+@pytest.mark.parametrize("axis,ang", [((0, 0, 1), 0.9), ((1, 0.3, -0.2), 1.3), ((0.2, 1, 0.5), -2.1)])
+def test_rotation_invariance(ora, axis, ang):
+    """C_ext and C_sca of a cluster, and C_sca,SH of a single sphere, are
+    scalars: rotating the spheres and the incident wave together leaves them
+    unchanged.  This holds only if the translation coefficients, the
+    plane-wave coefficients, the Clebsch-Gordan tables and the second-harmonic
+    T-matrices carry the right m-dependence.  (C_sca,SH of a cluster is not
+    invariant in the reference's formulation: its source is built from
+    conj(x_int), PreconditionedMatrixSolver.h:65-66, see
+    test_sh_source_transformation_law.)  The incident direction stays away from
+    the poles (the 1e-6 shift of AuxCoefficients.cpp:228 is not covariant)."""
+    from optimet_amd.problem import drude_eps
+    nm = 1e-9
+    e1, e2 = drude_eps(750 * nm)
+    cl = Cluster(xyz=np.array([[0, 0, 0], [150, 40, -30], [-20, 130, 90]], float) * nm,
+                 radius=np.array([50, 40, 45], float) * nm, eps_r=np.full(3, e1), eps_r_sh=np.full(3, e2), nmax=8,
+                 wavelength=750 * nm, theta_k=1.1, phi_k=0.4, etheta=0.8, ephi=0.6)
+    one = Cluster(xyz=np.zeros((1, 3)), radius=np.array([55 * nm]), eps_r=np.array([e1]), eps_r_sh=np.array([e2]),
+                  nmax=6, wavelength=750 * nm, ksippp=np.array([1e-18 + 0j]), ksiparppar=np.array([3e-19 + 0j]),
+                  gamma=np.array([1e-19 + 0j]), sh=True, theta_k=1.1, phi_k=0.4, etheta=0.8, ephi=0.6)
+    R = _rot(axis, ang)
+    for c, keys in ((cl, ("cext", "csca")), (one, ("cext", "csca", "csca_sh"))):
+        r0 = ora.solve_all(c)
+        c2 = _rotate_problem(c, R)
+        assert 0.2 < c2.theta_k < np.pi - 0.2
+        r1 = ora.solve_all(c2)
+        for key in keys:
+            assert abs(r1[key] - r0[key]) / abs(r0[key]) < 1e-9, key
+
+
+def test_sh_source_transformation_law(ora):
+    """The second-harmonic source of Geometry::getIncLocalSH is bilinear in the
+    coefficients handed to it with the Clebsch-Gordan rule M = m1 + m2, and
+    the reference hands it conj(x_int) (PreconditionedMatrixSolver.h:65-66):
+    under a rotation about z by a, x_int[n, m] -> x_int[n, m] e^{-i m a}, the
+    source must therefore become K[n, M] e^{+i M a} for any x_int (a wrong
+    table index, a wrong sign of m or a missing conjugate breaks this)."""
+    from optimet_amd.problem import drude_eps
+    nm = 1e-9
+    e1, e2 = drude_eps(800 * nm)
+    nmax = 4
+    c = Cluster(xyz=np.zeros((1, 3)), radius=np.array([60 * nm]), eps_r=np.array([e1]), eps_r_sh=np.array([e2]),
+                nmax=nmax, wavelength=800 * nm, ksippp=np.array([1e-18 + 0j]), ksiparppar=np.array([2e-19 + 0j]),
+                gamma=np.array([1e-19 + 0j]), sh=True)
+    tabs = ora.cg_tables(nmax)
+    ms = np.array([m for n in range(1, nmax + 1) for m in range(n, -n - 1, -1)] * 2)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(ms.size) + 1j * rng.standard_normal(ms.size)
+    K, K1 = ora.source_sh(c, tabs, x)
+    for a in (0.7, -2.3):
+        Kr, K1r = ora.source_sh(c, tabs, x * np.exp(-1j * ms * a))
+        assert np.linalg.norm(Kr - K * np.exp(1j * ms * a)) < 1e-13 * np.linalg.norm(K)
+        assert np.linalg.norm(K1r - K1 * np.exp(1j * ms * a)) < 1e-13 * np.linalg.norm(K1)
+    # bilinear: scaling x by s scales K by s^2 (conj(s)^2 for the conjugated input)
+    K2, _ = ora.source_sh(c, tabs, 2.0 * x)
+    assert np.linalg.norm(K2 - 4.0 * K) < 1e-13 * np.linalg.norm(K)
+
+
+def test_golden_fixtures(ora):
+    """The oracle reproduces the committed fixtures (tests/golden/make_golden.py)."""
+    import os
+    from tests.golden.make_golden import CASES
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    for name, build in CASES.items():
+        ref = np.load(os.path.join(here, name + ".npz"))
+        out = ora.solve_all(build())
+        for key in ref.files:
+            got = np.asarray(out[key])
+            assert np.linalg.norm(got - ref[key]) <= 1e-12 * np.linalg.norm(ref[key]), (name, key)
