@@ -39,3 +39,18 @@ def test_no_gpu_fails_loudly_not_silently():
             "except solver.OmxError as e:\n    print('ERR', e.code)\n")
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT, timeout=120)
     assert "ERR -2" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("compiler,lang", [("gcc", "c"), ("g++", "c++")])
+def test_header_compiles(tmp_path, compiler, lang):
+    """include/optimet_mi355x.h is plain C and C++ (what the adapter includes)."""
+    import shutil
+    import subprocess
+    if shutil.which(compiler) is None:
+        pytest.skip(f"{compiler} not found")
+    src = tmp_path / ("t." + ("c" if lang == "c" else "cpp"))
+    src.write_text('#include "optimet_mi355x.h"\nint main(void) { omx_stats s; (void)s; return OMX_OK; }\n')
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    r = subprocess.run([compiler, "-Wall", "-Werror", "-fsyntax-only", "-I", inc, str(src)], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
