@@ -1027,94 +1027,225 @@ __global__ void gather_kernel(const cplx *b, const int *perm, cplx *y, int Np) {
   if(i < Np) y[i] = b[perm[i]];
 }
 
-// Sync-free blocked triangular solve.  Workgroup k owns block row k: it
-// accumulates the products of its off-diagonal tiles with the blocks x_b as
-// their flags are published, then applies the stored inverse of its diagonal
-// block (unit-lower L11^{-1} or U11^{-1}) as a 64 x 64 matrix-vector product.
-template <typename T, bool LOWER>
+// Sync-free blocked triangular solve.  Workgroup K owns SB consecutive
+// 64-row blocks (a super-block).  Far phase: it consumes the blocks x_b of
+// earlier super-blocks as their flags are published (agent-scope acquire,
+// bounded spin) and accumulates L[super-block rows, b] x_b, with the next
+// 64 x 16 tile slice of each thread prefetched into registers while the
+// current one is multiplied.  Near phase: the super-block's own blocks are
+// solved in sequence inside the workgroup -- the contributions of the blocks
+// just solved, then the stored inverse of the diagonal block -- so only every
+// SB-th link of the chain crosses workgroups.  With PRE the near phase's tiles
+// (diagonal-block inverses and the intra-super-block L tiles) are loaded into
+// registers at kernel start.  Thread t handles row r = t % 64 and columns
+// 16 (t / 64) .. 16 (t / 64) + 15 of every tile.
+template <typename T, bool LOWER, int SB, int PRE>
 __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size_t lda, int Np,
                                                    const T *__restrict__ dinv, cplx *x, int *flags, int epoch,
                                                    int *status) {
   constexpr int NB = kNB;
-  const int nblk = Np / NB;
-  const int k = LOWER ? blockIdx.x : nblk - 1 - blockIdx.x;
-  __shared__ cplx xs[NB];
+  constexpr int NI = SB * (SB - 1) / 2;  // intra-super-block tiles
+  const int nblk = Np / NB, nsb = (nblk + SB - 1) / SB;
+  const int K = LOWER ? blockIdx.x : nsb - 1 - blockIdx.x;
+  const int kb0 = K * SB, nb = min(SB, nblk - kb0);
+  const int t = threadIdx.x, r = t & 63;
+  const int cq = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform
+  constexpr int CH = 8;  // far blocks taken per round
+  __shared__ cplx xs[CH * NB];
   __shared__ cplx part[4][NB];
-  const int t = threadIdx.x, r = t & 63, cq = t >> 6;
-  // this thread's 16 entries of the inverse diagonal block: row r, columns 16 cq ..
-  const T *mi = dinv + (size_t)k * 2 * NB * NB + (LOWER ? 0 : NB * NB);
-  T mv[16];
+  __shared__ int nready_s;
+  __shared__ cplx rhs[SB][NB];
+  __shared__ cplx xsb[SB][NB];
+  // the thread's 16 entries of a 64 x 64 tile at base (column-major, leading
+  // dimension ld): a wave-uniform column address plus the lane's row offset
+  auto tile = [&](const T *base, size_t ld, T (&dst)[16]) {
+    const T *col = base + (size_t)(cq * 16) * ld;
 #pragma unroll
-  for(int cc = 0; cc < 16; ++cc) mv[cc] = mi[(size_t)(cq * 16 + cc) * NB + r];
-  cplx acc = mk(0.0);
-  const int ndep = LOWER ? k : nblk - 1 - k;
-  for(int ii = 0; ii < ndep; ++ii) {
-    const int b = LOWER ? ii : nblk - 1 - ii;
-    // issue this tile's loads before waiting for x_b
-    const T *a = A + (size_t)(b * NB + cq * 16) * lda + (size_t)k * NB + r;
-    T tv[16];
+    for(int cc = 0; cc < 16; ++cc) dst[cc] = col[(size_t)cc * ld + (unsigned)r];
+  };
+  // diagonal-block inverse of block k, and the tile L[k, kc] (column-major in A)
+  auto inv_base = [&](int k) { return dinv + (size_t)k * 2 * NB * NB + (LOWER ? 0 : NB * NB); };
+  auto a_base = [&](int k, int kc) { return A + (size_t)kc * NB * lda + (size_t)k * NB; };
+  // near-phase tiles: mv[s] = inverse of block kb0+s; lv[idx(s, s')] = L[kb0+s, kb0+s']
+  T mv[PRE ? SB : 1][16], lv[PRE ? (NI > 0 ? NI : 1) : 1][16];
+  auto preload = [&]() {
 #pragma unroll
-    for(int cc = 0; cc < 16; ++cc) tv[cc] = a[(size_t)cc * lda];
-    if(t == 0) {
+    for(int s = 0; s < SB; ++s)
+      if(s < nb) tile(inv_base(kb0 + s), NB, mv[PRE ? s : 0]);
+    int q = 0;
+#pragma unroll
+    for(int s = 0; s < SB; ++s)
+#pragma unroll
+      for(int s2 = 0; s2 < SB; ++s2) {
+        if(LOWER ? (s2 < s) : (s2 > s)) {
+          if(s < nb && s2 < nb) tile(a_base(kb0 + s, kb0 + s2), lda, lv[PRE ? q : 0]);
+          ++q;
+        }
+      }
+  };
+  if(PRE == 1) preload();
+  // ---- far phase
+  cplx acc[SB];
+#pragma unroll
+  for(int s = 0; s < SB; ++s) acc[s] = mk(0.0);
+  const int nfar = LOWER ? kb0 : nblk - (kb0 + nb);
+  T cur[16], nxt[16];
+  auto fblk = [&](int j) { return LOWER ? j : nblk - 1 - j; };  // j-th far block in order of publication
+  if(nfar > 0) tile(a_base(kb0, fblk(0)), lda, cur);
+  for(int j0 = 0; j0 < nfar;) {
+    // how many of the next CH far blocks are published (wave 0, one flag per lane)
+    if(t < 64) {
       unsigned spins = 0;
-      while(__hip_atomic_load(&flags[b], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch &&
-            __hip_atomic_load(&status[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+      int nr = 0;
+      while(true) {
+        const bool ok = t < CH && j0 + t < nfar &&
+                        __hip_atomic_load(&flags[fblk(j0 + t)], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+        const unsigned long long m = __ballot(ok);
+        nr = __builtin_ctzll(~m);  // leading run of published blocks
+        if(nr > 0 || __hip_atomic_load(&status[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
         if(++spins > (1u << 24)) {
-          __hip_atomic_store(&status[1], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if(t == 0) __hip_atomic_store(&status[1], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
       }
+      if(t == 0) nready_s = nr > 0 ? nr : nfar - j0;  // on a timeout: drain (the host reports the error)
     }
     __syncthreads();
-    // x_b was published with a release before its flag: read it with
-    // agent-scope atomic loads instead of invalidating the caches
-    if(t < NB) {
-      const unsigned long long *src = reinterpret_cast<const unsigned long long *>(x + (size_t)b * NB + t);
+    const int nr = nready_s;
+    // those blocks of x were published with a release before their flags:
+    // read them with agent-scope atomic loads instead of invalidating caches
+    for(int idx = t; idx < nr * NB; idx += 256) {
+      const unsigned long long *src =
+          reinterpret_cast<const unsigned long long *>(x + (size_t)fblk(j0 + idx / NB) * NB + (idx % NB));
       unsigned long long re = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       unsigned long long im = __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      xs[t] = mk(__longlong_as_double((long long)re), __longlong_as_double((long long)im));
+      xs[idx] = mk(__longlong_as_double((long long)re), __longlong_as_double((long long)im));
     }
     __syncthreads();
+    for(int c = 0; c < nr; ++c) {
+      const int j = j0 + c, b = fblk(j);
 #pragma unroll
-    for(int cc = 0; cc < 16; ++cc) {
-      T v = tv[cc];
-      cplx xv = xs[cq * 16 + cc];
-      acc.x += (double)v.x * xv.x - (double)v.y * xv.y;
-      acc.y += (double)v.x * xv.y + (double)v.y * xv.x;
+      for(int s = 0; s < SB; ++s) {
+        if(s < nb) {
+          // prefetch the next slice: (s + 1, b) or (0, next far block)
+          const bool inner = s + 1 < nb;
+          if(inner || j + 1 < nfar) tile(inner ? a_base(kb0 + s + 1, b) : a_base(kb0, fblk(j + 1)), lda, nxt);
+#pragma unroll
+          for(int cc = 0; cc < 16; ++cc) {
+            const T v = cur[cc];
+            const cplx xv = xs[c * NB + cq * 16 + cc];
+            acc[s].x += (double)v.x * xv.x - (double)v.y * xv.y;
+            acc[s].y += (double)v.x * xv.y + (double)v.y * xv.x;
+          }
+#pragma unroll
+          for(int cc = 0; cc < 16; ++cc) cur[cc] = nxt[cc];
+        }
+      }
     }
     __syncthreads();
+    j0 += nr;
   }
-  part[cq][r] = acc;
-  __syncthreads();
-  if(t < NB) xs[t] = x[(size_t)k * NB + t] - (part[0][t] + part[1][t] + part[2][t] + part[3][t]);
-  __syncthreads();
-  cplx y = mk(0.0);
+  if(PRE == 2) preload();
+  // right-hand sides of the super-block's blocks minus the far contributions
 #pragma unroll
-  for(int cc = 0; cc < 16; ++cc) {
-    T v = mv[cc];
-    cplx sv = xs[cq * 16 + cc];
-    y.x += (double)v.x * sv.x - (double)v.y * sv.y;
-    y.y += (double)v.x * sv.y + (double)v.y * sv.x;
+  for(int s = 0; s < SB; ++s) {
+    if(s < nb) {
+      part[cq][r] = acc[s];
+      __syncthreads();
+      if(t < NB) rhs[s][t] = x[(size_t)(kb0 + s) * NB + t] - (part[0][t] + part[1][t] + part[2][t] + part[3][t]);
+      __syncthreads();
+    }
   }
-  part[cq][r] = y;
-  __syncthreads();
-  if(t < NB) x[(size_t)k * NB + t] = part[0][t] + part[1][t] + part[2][t] + part[3][t];
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  if(t == 0) __hip_atomic_store(&flags[k], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  // ---- near phase: the super-block's blocks in order
+#pragma unroll
+  for(int i = 0; i < SB; ++i) {
+    const int s = LOWER ? i : SB - 1 - i;  // compile-time index; blocks past nb are skipped
+    if(s < nb) {
+      const int k = kb0 + s;
+      // contributions of the blocks of this super-block solved before
+      cplx a = mk(0.0);
+      int q = 0;
+#pragma unroll
+      for(int ss = 0; ss < SB; ++ss)
+#pragma unroll
+        for(int s2 = 0; s2 < SB; ++s2) {
+          if(LOWER ? (s2 < ss) : (s2 > ss)) {
+            if(ss == s && s2 < nb) {
+              T lt[16];
+              if(PRE) {
+#pragma unroll
+                for(int cc = 0; cc < 16; ++cc) lt[cc] = lv[PRE ? q : 0][cc];
+              } else {
+                tile(a_base(k, kb0 + s2), lda, lt);
+              }
+#pragma unroll
+              for(int cc = 0; cc < 16; ++cc) {
+                const cplx xv = xsb[s2][cq * 16 + cc];
+                a.x += (double)lt[cc].x * xv.x - (double)lt[cc].y * xv.y;
+                a.y += (double)lt[cc].x * xv.y + (double)lt[cc].y * xv.x;
+              }
+            }
+            ++q;
+          }
+        }
+      part[cq][r] = a;
+      __syncthreads();
+      if(t < NB) xs[t] = rhs[s][t] - (part[0][t] + part[1][t] + part[2][t] + part[3][t]);
+      __syncthreads();
+      // x_k = (diagonal block)^{-1} r
+      T m[16];
+      if(PRE) {
+#pragma unroll
+        for(int cc = 0; cc < 16; ++cc) m[cc] = mv[PRE ? s : 0][cc];
+      } else {
+        tile(inv_base(k), NB, m);
+      }
+      cplx y = mk(0.0);
+#pragma unroll
+      for(int cc = 0; cc < 16; ++cc) {
+        const cplx sv = xs[cq * 16 + cc];
+        y.x += (double)m[cc].x * sv.x - (double)m[cc].y * sv.y;
+        y.y += (double)m[cc].x * sv.y + (double)m[cc].y * sv.x;
+      }
+      part[cq][r] = y;
+      __syncthreads();
+      if(t < NB) {
+        const cplx xk = part[0][t] + part[1][t] + part[2][t] + part[3][t];
+        xsb[s][t] = xk;
+        x[(size_t)k * NB + t] = xk;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __syncthreads();
+      if(t == 0) __hip_atomic_store(&flags[k], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 }
+
+// Super-block size and near-phase prefetch of the triangular solves (the
+// complex-FP64 tiles of a prefetch would not fit in the register file).
+template <typename T> struct TrsvCfg;
+template <> struct TrsvCfg<float2> {
+  static constexpr int SB = 2;
+  static constexpr int PRE = 1;
+};
+template <> struct TrsvCfg<double2> {
+  static constexpr int SB = 2;
+  static constexpr int PRE = 2;
+};
 
 template <typename T>
 void lu_solve(const T *A, int Np, size_t lda, LuWork &w, const cplx *b, cplx *x, hipStream_t st) {
   const T *dinv = reinterpret_cast<const T *>(w.dinv);
+  constexpr int SB = TrsvCfg<T>::SB;
+  constexpr int PRE = TrsvCfg<T>::PRE;
   hipLaunchKernelGGL(gather_kernel, dim3((Np + 255) / 256), dim3(256), 0, st, b, w.perm, x, Np);
-  int nblk = Np / kNB;
+  const int nblk = Np / kNB, nsb = (nblk + SB - 1) / SB;
   int e1 = ++w.epoch;
-  hipLaunchKernelGGL((trsv_kernel<T, true>), dim3(nblk), dim3(256), 0, st, A, lda, Np, dinv, x, w.flags, e1,
+  hipLaunchKernelGGL((trsv_kernel<T, true, SB, PRE>), dim3(nsb), dim3(256), 0, st, A, lda, Np, dinv, x, w.flags, e1,
                      w.status);
   int e2 = ++w.epoch;
-  hipLaunchKernelGGL((trsv_kernel<T, false>), dim3(nblk), dim3(256), 0, st, A, lda, Np, dinv, x, w.flags, e2,
+  hipLaunchKernelGGL((trsv_kernel<T, false, SB, PRE>), dim3(nsb), dim3(256), 0, st, A, lda, Np, dinv, x, w.flags, e2,
                      w.status);
 }
 
