@@ -81,6 +81,7 @@ typedef struct {
   double t_lu_panel_ms;  /* panel work of each pair up to its far-column step */
   double t_factor_total_ms; /* whole factorisations */
   int pivot_fallbacks;   /* refactorisations with partial pivoting since creation */
+  int sweep_batch;       /* systems per batch of the last omx_solve_sweep */
 } omx_stats;
 
 const char *omx_last_error(void);
@@ -153,6 +154,21 @@ int omx_set_profile(omx_context *ctx, int on);
  * field of every object) to host memory. */
 int omx_read_matrix(omx_context *ctx, int which, double *out);
 int omx_read_vector(omx_context *ctx, int which, double *out);
+
+/* Wavelength sweep, batched: the current geometry and excitation solved at
+ * nl vacuum wavelengths (m), as Simulation::scan_wavelengths does one
+ * wavelength at a time (Simulation.cpp:604-685, update() and solve() per
+ * wavelength).  eps_r and eps_r_sh (nl*P complex, wavelength-major) give each
+ * sphere's relative permittivity at each wavelength (at 2 omega for eps_r_sh);
+ * NULL keeps the geometry's values.  Up to `batch` systems (0: automatic, at
+ * most 64 and half the free device memory) are assembled, factorised (block
+ * pivoting) and refined together; a system with a zero pivot or a refinement
+ * that does not converge is redone alone with partial pivoting.  Outputs (any
+ * may be NULL): x_sca, x_int, x_sca_sh, x_int_sh nl*N complex each, xs nl*4
+ * cross sections as omx_cross_sections, iterations nl*2 (omega, 2 omega). */
+int omx_solve_sweep(omx_context *ctx, int nl, const double *lambdas, const double *eps_r, const double *eps_r_sh,
+                    int batch, double *x_sca, double *x_int, double *x_sca_sh, double *x_int_sh, double *xs,
+                    int *iterations);
 
 /* Stand-alone dense complex solve A x = b on the device (A column-major,
  * N x N); kind OMX_SOLVER_LU64 or OMX_SOLVER_LU32_IR.  Uses the context's

@@ -105,6 +105,11 @@ struct omx_context {
   int prof_launches = 0;
   DBuf d_piv2, d_perm2, d_flags2;
   DBuf d_dinv, d_dinv2, d_F64, d_F64sh;  // diagonal-block inverses; FP64 factors (the FP64 matrices stay intact)
+  // wavelength sweeps: per-system buffers of one batch ([system][...])
+  struct Sweep {
+    DBuf S64, S32, V64, V32, F64, F64sh, eps, eps_sh, ov, incloc, Q, w, x, xint, xsh, xintsh, K, K1, cross;
+    DBuf perm, perm2, flags, flags2, dinv, dinv2, map, status, basis, r, w1, w2, gpart, part, hd, nrm, sc, act;
+  } sw;
   omx_stats stats{};
   double omega() const { return omx::kC0 * (2.0 * omx::kPi / lambda); }
   cplx waveK() const {
@@ -473,6 +478,429 @@ void solve_system(omx_context *c, bool shf, const cplx *rhs, cplx *x, int *iters
   }
 }
 
+// ------------------------------------------------------------------ batches
+// Restarted GMRES for nb systems in lockstep (the Gmres above, with every
+// vector operation batched): each system keeps its own Hessenberg matrix,
+// rotations, residual and stopping state; a system that has converged (or
+// stalled) in a cycle has its further basis vectors scaled to zero and its
+// solution left alone.  prec(in, out) applies the batched preconditioner.
+struct GmresB {
+  omx_context *c;
+  const cplx *A;  // FP64 operators, sA apart
+  size_t sA;
+  int Np, nb;
+  std::function<void(const cplx *, cplx *)> prec;
+  omx_context::Sweep &w;
+  size_t sv() const { return (size_t)Np; }
+  std::vector<double> nrms(const cplx *x) {
+    omx::nrm2sq_b(x, sv(), Np, nb, w.nrm.as<double>() + nb, w.nrm.as<double>(), c->st);
+    std::vector<double> v(nb);
+    HIPCHK(hipMemcpyAsync(v.data(), w.nrm.p, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    for(double &e : v) e = std::sqrt(e);
+    return v;
+  }
+  void matvec(const cplx *x, cplx *y) { omx::gemv64(A, (size_t)Np, Np, x, y, w.gpart.as<cplx>(), c->st, nb, sA, sv()); }
+  // per-system scalars a[z], b[z] in device memory
+  void scale(cplx *y, size_t sy, const std::vector<cplx> &a, const cplx *x, size_t sx, const std::vector<cplx> &b,
+             const int *act) {
+    cplx *sc = w.sc.as<cplx>();
+    std::vector<cplx> h(2 * nb);
+    for(int z = 0; z < nb; ++z) {
+      h[z] = a[z];
+      h[nb + z] = b[z];
+    }
+    HIPCHK(hipMemcpyAsync(sc, h.data(), sizeof(cplx) * 2 * nb, hipMemcpyHostToDevice, c->st));
+    omx::vaxpby_b(y, sy, sc, x, sx, sc + nb, Np, nb, act, c->st);
+  }
+  // Solves A_z x_z = b_z (x holds the initial guesses); res[z] the final
+  // relative residual, its[z] the iterations spent on system z.
+  void solve(const cplx *b, cplx *x, double tol, int maxit, int m, std::vector<double> &res, std::vector<int> &its) {
+    // basis vector k of system z at V + (k nb + z) Np: the systems' k-th
+    // vectors are contiguous, as the batched preconditioner expects
+    cplx *V = w.basis.as<cplx>();
+    const size_t sV = (size_t)Np, ld = (size_t)Np * nb;
+    cplx *r = w.r.as<cplx>(), *wv = w.w1.as<cplx>(), *zv = w.w2.as<cplx>();
+    cplx *hd = w.hd.as<cplx>(), *part = w.part.as<cplx>();
+    const int sh = m + 2;
+    int *act_d = w.act.as<int>();
+    res.assign(nb, 0.0);
+    its.assign(nb, 0);
+    const std::vector<double> bn = nrms(b);
+    std::vector<int> active(nb), stall(nb, 0);
+    std::vector<double> prev(nb, 0.0);
+    for(int z = 0; z < nb; ++z) active[z] = bn[z] > 0.0;
+    const cplx zero = omx::mk(0.0), one = omx::mk(1.0);
+    {  // zero right-hand side: zero solution
+      std::vector<cplx> a(nb, zero), bb(nb);
+      bool any = false;
+      for(int z = 0; z < nb; ++z) {
+        bb[z] = active[z] ? one : zero;
+        any |= !active[z];
+      }
+      if(any) scale(x, sv(), a, x, sv(), bb, nullptr);
+    }
+    using Z = std::complex<double>;
+    std::vector<Z> H((size_t)nb * (m + 1) * m), g((size_t)nb * (m + 1)), cs((size_t)nb * m), sn((size_t)nb * m);
+    std::vector<int> jz(nb);
+    int it = 0;
+    while(true) {
+      matvec(x, r);
+      omx::vsub_b(r, b, r, Np, nb, sv(), c->st);
+      const std::vector<double> beta = nrms(r);
+      bool any = false;
+      for(int z = 0; z < nb; ++z) {
+        if(!active[z]) continue;
+        res[z] = beta[z] / bn[z];
+        if(res[z] <= tol || its[z] >= maxit || !(res[z] == res[z])) {
+          active[z] = 0;
+          continue;
+        }
+        if(prev[z] > 0.0 && beta[z] > 0.5 * prev[z]) {
+          if(++stall[z] >= 2) {
+            active[z] = 0;
+            continue;
+          }
+        } else {
+          stall[z] = 0;
+        }
+        prev[z] = beta[z];
+        any = true;
+      }
+      if(!any || it >= maxit) break;
+      std::vector<int> live(active);  // systems still building this cycle's basis
+      {
+        std::vector<cplx> a(nb, zero), bb(nb, zero);
+        for(int z = 0; z < nb; ++z) a[z] = live[z] ? omx::mk(1.0 / beta[z]) : zero;
+        scale(V, sV, a, r, sv(), bb, nullptr);  // V_0 of every system
+      }
+      for(int z = 0; z < nb; ++z) {
+        std::fill(g.begin() + (size_t)z * (m + 1), g.begin() + (size_t)(z + 1) * (m + 1), Z(0.0));
+        g[(size_t)z * (m + 1)] = beta[z];
+        jz[z] = 0;
+      }
+      for(int j = 0; j < m && it < maxit; ++j, ++it) {
+        const cplx *vj = V + (size_t)j * ld;
+        prec(vj, zv);
+        matvec(zv, wv);
+        std::vector<cplx> h1((size_t)nb * sh), h2((size_t)nb * sh);
+        for(int pass = 0; pass < 2; ++pass) {
+          omx::dotc_multi_b(V, ld, sV, j + 1, wv, sv(), Np, nb, part, hd, sh, c->st);
+          omx::axpy_multi_b(V, ld, sV, j + 1, hd, sh, wv, sv(), Np, nb, c->st);
+          HIPCHK(hipMemcpyAsync(pass ? h2.data() : h1.data(), hd, sizeof(cplx) * nb * sh, hipMemcpyDeviceToHost,
+                                c->st));
+        }
+        const std::vector<double> hn = nrms(wv);
+        {
+          std::vector<cplx> a(nb, zero), bb(nb, zero);
+          for(int z = 0; z < nb; ++z) a[z] = (live[z] && hn[z] > 0.0) ? omx::mk(1.0 / hn[z]) : zero;
+          scale(V + (size_t)(j + 1) * ld, sV, a, wv, sv(), bb, nullptr);
+        }
+        bool any_live = false;
+        for(int z = 0; z < nb; ++z) {
+          if(!live[z]) continue;
+          Z *hc = &H[((size_t)z * m + j) * (m + 1)];
+          for(int i = 0; i <= j; ++i)
+            hc[i] = Z(h1[(size_t)z * sh + i].x + h2[(size_t)z * sh + i].x, h1[(size_t)z * sh + i].y + h2[(size_t)z * sh + i].y);
+          hc[j + 1] = hn[z];
+          Z *csz = &cs[(size_t)z * m], *snz = &sn[(size_t)z * m], *gz = &g[(size_t)z * (m + 1)];
+          for(int i = 0; i < j; ++i) {
+            Z t = csz[i] * hc[i] + snz[i] * hc[i + 1];
+            hc[i + 1] = -std::conj(snz[i]) * hc[i] + csz[i] * hc[i + 1];
+            hc[i] = t;
+          }
+          const double den = std::sqrt(std::norm(hc[j]) + std::norm(hc[j + 1]));
+          if(den == 0.0) {
+            csz[j] = 1.0;
+            snz[j] = 0.0;
+          } else {
+            const Z ph = std::abs(hc[j]) > 0 ? hc[j] / std::abs(hc[j]) : Z(1.0);
+            csz[j] = std::abs(hc[j]) / den;
+            snz[j] = ph * std::conj(hc[j + 1]) / den;
+          }
+          hc[j] = csz[j] * hc[j] + snz[j] * hc[j + 1];
+          hc[j + 1] = 0.0;
+          gz[j + 1] = -std::conj(snz[j]) * gz[j];
+          gz[j] = csz[j] * gz[j];
+          jz[z] = j + 1;
+          ++its[z];
+          if(std::abs(gz[j + 1]) <= 0.5 * tol * bn[z] || hn[z] == 0.0 || its[z] >= maxit)
+            live[z] = 0;  // its later basis vectors are zero
+          else
+            any_live = true;
+        }
+        if(!any_live) {
+          ++it;
+          break;
+        }
+      }
+      // y_z = H_z^{-1} g_z, x_z += M^{-1} V_z y_z
+      int jmax = 0;
+      std::vector<cplx> yh((size_t)nb * sh, zero);
+      for(int z = 0; z < nb; ++z) {
+        if(!active[z]) continue;
+        const int jj = jz[z];
+        jmax = std::max(jmax, jj);
+        std::vector<Z> y(jj);
+        for(int i = jj - 1; i >= 0; --i) {
+          Z sacc = g[(size_t)z * (m + 1) + i];
+          for(int k = i + 1; k < jj; ++k) sacc -= H[((size_t)z * m + k) * (m + 1) + i] * y[k];
+          y[i] = sacc / H[((size_t)z * m + i) * (m + 1) + i];
+        }
+        for(int i = 0; i < jj; ++i) yh[(size_t)z * sh + i] = omx::mk(y[i].real(), y[i].imag());
+      }
+      if(jmax == 0) break;
+      HIPCHK(hipMemcpyAsync(hd, yh.data(), sizeof(cplx) * nb * sh, hipMemcpyHostToDevice, c->st));
+      omx::gemv_multi_b(V, ld, sV, jmax, hd, sh, zv, sv(), Np, nb, c->st);
+      prec(zv, wv);
+      std::vector<int> act_h(active);
+      HIPCHK(hipMemcpyAsync(act_d, act_h.data(), sizeof(int) * nb, hipMemcpyHostToDevice, c->st));
+      scale(x, sv(), std::vector<cplx>(nb, one), wv, sv(), std::vector<cplx>(nb, one), act_d);
+      // a correction below the FP64 resolution of x ends that system's refinement
+      const std::vector<double> dn = nrms(wv), xn = nrms(x);
+      for(int z = 0; z < nb; ++z)
+        if(active[z] && dn[z] <= 1e-16 * xn[z]) active[z] = 0;
+    }
+  }
+};
+
+// Buffers of a batch of nb systems of order Np (mixed: FP32 copies too).
+void sweep_alloc(omx_context *c, int nb, bool f64, bool sh) {
+  auto &w = c->sw;
+  size_t *T = &c->total;
+  const size_t Np = c->Np, n2 = Np * Np, P = c->P, S = c->nmax + 1;
+  const int m = c->restart;
+  w.S64.ensure(sizeof(cplx) * n2 * nb, T);
+  if(f64) {
+    w.F64.ensure(sizeof(cplx) * n2 * nb, T);
+    w.S32.release(T);
+  } else {
+    w.S32.ensure(sizeof(float2) * n2 * nb, T);
+    w.F64.release(T);
+  }
+  if(sh) {
+    w.V64.ensure(sizeof(cplx) * n2 * nb, T);
+    if(f64) w.F64sh.ensure(sizeof(cplx) * n2 * nb, T); else w.V32.ensure(sizeof(float2) * n2 * nb, T);
+  }
+  w.eps.ensure(sizeof(cplx) * P * nb, T);
+  w.eps_sh.ensure(sizeof(cplx) * P * nb, T);
+  w.ov.ensure(sizeof(cplx) * omx::OV_COUNT * P * 2 * S * nb, T);
+  for(DBuf *b : {&w.incloc, &w.Q, &w.x, &w.xint, &w.xsh, &w.xintsh, &w.K, &w.K1, &w.r, &w.w1, &w.w2})
+    b->ensure(sizeof(cplx) * Np * nb, T);
+  w.w.ensure(sizeof(double) * 2 * P * c->nf * nb, T);
+  w.cross.ensure(sizeof(double) * 4 * P * nb, T);
+  for(DBuf *b : {&w.perm, &w.perm2}) b->ensure(sizeof(int) * Np * nb, T);
+  for(DBuf *b : {&w.flags, &w.flags2}) b->ensure(sizeof(int) * (Np / omx::kNB + 1) * nb, T);
+  for(DBuf *b : {&w.dinv, &w.dinv2}) b->ensure(sizeof(cplx) * 2 * omx::kNB * Np * nb, T);
+  w.map.ensure(sizeof(int) * 3 * (Np + 2 * omx::kNB) * nb, T);
+  w.status.ensure(sizeof(int) * 4 * nb, T);
+  w.basis.ensure(sizeof(cplx) * Np * (m + 1) * nb, T);
+  w.gpart.ensure(sizeof(cplx) * Np * omx::kGemvSplitMax * nb, T);
+  w.part.ensure(sizeof(cplx) * 128 * (m + 2) * nb, T);
+  w.hd.ensure(sizeof(cplx) * (m + 2) * nb, T);
+  w.nrm.ensure(sizeof(double) * 160 * nb, T);
+  w.sc.ensure(sizeof(cplx) * 2 * nb, T);
+  w.act.ensure(sizeof(int) * nb, T);
+}
+
+omx::LuWork sweep_lu(omx_context *c, int nb, bool second) {
+  auto &w = c->sw;
+  const int Np = c->Np;
+  omx::LuWork lw{};
+  lw.perm = (second ? w.perm2 : w.perm).as<int>();
+  lw.piv = lw.perm;  // unused by block pivoting
+  lw.flags = (second ? w.flags2 : w.flags).as<int>();
+  lw.dinv = (second ? w.dinv2 : w.dinv).p;
+  const int ms = Np + 2 * omx::kNB;
+  lw.mdst = w.map.as<int>();
+  lw.msrc = w.map.as<int>() + (size_t)ms * nb;
+  lw.mcnt = w.map.as<int>() + 2 * (size_t)ms * nb;
+  lw.status = w.status.as<int>();
+  lw.wlu = c->d_wlu.p;
+  lw.aux = c->aux;
+  lw.ev_next = c->lu_ev[0];
+  lw.ev_piv = c->lu_ev[1];
+  lw.ev_bulk = c->lu_ev[2];
+  lw.ev_pan = c->lu_ev[3];
+  lw.gmax = 0;
+  lw.epoch = 0;
+  lw.pivot = omx::kPivotBlock;
+  lw.nb = nb;
+  lw.bs.a = (size_t)Np * Np;
+  lw.bs.inv = (size_t)2 * omx::kNB * Np;
+  lw.bs.vec = (size_t)Np;
+  lw.bs.perm = Np;
+  lw.bs.map = ms;
+  lw.bs.cnt = ms;
+  lw.bs.flags = Np / omx::kNB + 1;
+  lw.bs.status = 4;
+  return lw;
+}
+
+// One batch of a wavelength sweep: nb systems (wavelengths lam[z], absolute
+// permittivities eps[z * P + i], eps_sh[z * P + i]) assembled, factorised and
+// refined side by side.  Systems whose factorisation hit a zero pivot or whose
+// refinement did not converge are flagged in bad[z] for the caller to redo
+// one at a time with partial pivoting.
+void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, const cplx *eps_sh, double *x_sca,
+                 double *x_int, double *x_sca_sh, double *x_int_sh, double *xs, std::vector<int> &bad,
+                 std::vector<int> &iters, std::vector<double> &resid) {
+  const bool f64 = c->kind == OMX_SOLVER_LU64, sh = c->sh;
+  sweep_alloc(c, nb, f64, sh);
+  auto &w = c->sw;
+  const int P = c->P, nmax = c->nmax, N = c->N, Np = c->Np;
+  const size_t n2 = (size_t)Np * Np, lda = (size_t)Np;
+  const size_t ovs = (size_t)omx::OV_COUNT * P * 2 * (nmax + 1), ws = (size_t)2 * P * c->nf;
+  hipStream_t st = c->st;
+  HIPCHK(hipMemcpyAsync(w.eps.p, eps, sizeof(cplx) * P * nb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(w.eps_sh.p, eps_sh, sizeof(cplx) * P * nb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(w.flags.p, 0, w.flags.bytes, st));
+  HIPCHK(hipMemsetAsync(w.flags2.p, 0, w.flags2.bytes, st));
+  HIPCHK(hipMemsetAsync(w.status.p, 0, w.status.bytes, st));
+  HIPCHK(hipMemsetAsync(w.Q.p, 0, w.Q.bytes, st));
+  HIPCHK(hipMemsetAsync(c->d_status.p, 0, 16, st));
+  const cplx bge = kEps0 * c->bg_eps_r, bgm = kMu0 * c->bg_mu_r;
+  std::complex<double> bgn = std::sqrt(std::complex<double>(c->bg_eps_r.x, c->bg_eps_r.y) *
+                                       std::complex<double>(c->bg_mu_r.x, c->bg_mu_r.y));
+  // plane-wave coefficients do not depend on the wavelength (Excitation.cpp:48-75)
+  omx::IncArgs ia{nmax, c->theta, c->phi, c->eth, c->eph, c->d_inc.as<cplx>()};
+  omx::run_incident(ia, st);
+  cplx *cS64 = w.S64.as<cplx>(), *cV64 = w.V64.as<cplx>();
+  float2 *cS32 = w.S32.as<float2>(), *cV32 = w.V32.as<float2>();
+  for(int z = 0; z < nb; ++z) {
+    const double k0 = 2.0 * omx::kPi / lam[z], omega = omx::kC0 * k0;
+    const cplx k = omx::mk(k0 * bgn.real(), k0 * bgn.imag());
+    const cplx *ez = w.eps.as<cplx>() + (size_t)z * P, *esz = w.eps_sh.as<cplx>() + (size_t)z * P;
+    cplx *ov = w.ov.as<cplx>() + z * ovs;
+    omx::ObjArgs oa{P, nmax, omega, bge, bgm, c->d_xyz.as<double>(), c->d_radius.as<double>(), ez,
+                    c->d_mu.as<cplx>(), esz, c->d_sph.as<double>(), ov};
+    omx::run_objects(oa, st);
+    omx::OriginArgs og{P, nmax, k, c->d_sph.as<double>(), ov, c->d_coef.as<double>(), c->d_inc.as<cplx>(),
+                       w.incloc.as<cplx>() + z * lda, w.Q.as<cplx>() + z * lda, w.w.as<double>() + z * ws};
+    omx::run_origin(og, st);
+    omx::PairArgs pa{P, nmax, omx::OV_T, k, c->d_sph.as<double>(), ov, c->d_coef.as<double>(), lda,
+                     cS64 + z * n2, f64 ? nullptr : cS32 + z * n2};
+    omx::run_pairs(pa, st);
+    omx::run_pad64(cS64 + z * n2, N, Np, lda, st);
+    if(!f64) omx::run_pad32(cS32 + z * n2, N, Np, lda, st);
+    if(sh) {
+      omx::PairArgs pv = pa;
+      pv.tkind = omx::OV_TSH;
+      pv.k = omx::mk(2.0 * k.x, 2.0 * k.y);  // PreconditionedMatrix.cpp:592
+      pv.S64 = cV64 + z * n2;
+      pv.S32 = f64 ? nullptr : cV32 + z * n2;
+      omx::run_pairs(pv, st);
+      omx::run_pad64(cV64 + z * n2, N, Np, lda, st);
+      if(!f64) omx::run_pad32(cV32 + z * n2, N, Np, lda, st);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  auto status_scan = [&](std::vector<int> &flag) {
+    std::vector<int> h(4 * nb);
+    HIPCHK(hipMemcpyAsync(h.data(), w.status.p, sizeof(int) * 4 * nb, hipMemcpyDeviceToHost, st));
+    int g[2];
+    HIPCHK(hipMemcpyAsync(g, c->d_status.p, sizeof(g), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if(h[1] || g[1]) throw OmxError(OMX_ERR_DEVICE, "device wait timed out in a batched solve");
+    for(int z = 0; z < nb; ++z)
+      if(h[4 * z]) flag[z] = 1;
+  };
+  // solves (S or V) for every system; returns per-system convergence flags
+  auto solve_all = [&](bool second, const cplx *rhs, cplx *x) {
+    cplx *A64 = second ? cV64 : cS64;
+    omx::LuWork lw = sweep_lu(c, nb, second);
+    void *F = nullptr;
+    int rc;
+    if(f64) {
+      F = (second ? w.F64sh : w.F64).p;
+      HIPCHK(hipMemcpyAsync(F, A64, sizeof(cplx) * n2 * nb, hipMemcpyDeviceToDevice, st));
+      rc = omx::lu_factor<double2>(static_cast<double2 *>(F), Np, lda, lw, st, nullptr);
+    } else {
+      F = (second ? w.V32 : w.S32).p;
+      rc = omx::lu_factor<float2>(static_cast<float2 *>(F), Np, lda, lw, st, nullptr);
+    }
+    if(rc) throw OmxError(OMX_ERR_DEVICE, "batched lu_factor launch failed (" + std::to_string(rc) + ")");
+    status_scan(bad);
+    omx::LuWork *lwp = &lw;
+    GmresB gm{c, A64, n2, Np, nb, nullptr, w};
+    if(f64)
+      gm.prec = [c, F, lwp, Np, lda](const cplx *in, cplx *out) {
+        omx::lu_solve<double2>(static_cast<const double2 *>(F), Np, lda, *lwp, in, out, c->st);
+      };
+    else
+      gm.prec = [c, F, lwp, Np, lda](const cplx *in, cplx *out) {
+        omx::lu_solve<float2>(static_cast<const float2 *>(F), Np, lda, *lwp, in, out, c->st);
+      };
+    gm.prec(rhs, x);
+    std::vector<double> res;
+    std::vector<int> its;
+    gm.solve(rhs, x, c->tol, c->maxit, c->restart, res, its);
+    status_scan(bad);
+    for(int z = 0; z < nb; ++z) {
+      if(!(res[z] <= 1e3 * c->tol)) bad[z] = 1;
+      iters[2 * z + (second ? 1 : 0)] = its[z];
+      resid[2 * z + (second ? 1 : 0)] = res[z];
+    }
+  };
+  solve_all(false, w.Q.as<cplx>(), w.x.as<cplx>());
+  for(int z = 0; z < nb; ++z)
+    omx::vmul_ov(w.xint.as<cplx>() + z * lda, w.x.as<cplx>() + z * lda, w.ov.as<cplx>() + z * ovs, omx::OV_IAUX, P,
+                 nmax, nullptr, N, st);
+  if(sh) {
+    if(c->cg_nmax != nmax) {
+      size_t nf = (size_t)c->nf, sz = nf * nf * nf;
+      c->d_cg.ensure(sizeof(double) * 9 * sz, &c->total);
+      omx::run_cg(nmax, c->d_cg.as<double>(), st);
+      c->cg_nmax = nmax;
+    }
+    HIPCHK(hipMemsetAsync(w.K.p, 0, w.K.bytes, st));
+    for(int z = 0; z < nb; ++z) {
+      const double omega = omx::kC0 * (2.0 * omx::kPi / lam[z]);
+      omx::ShArgs sa{P, nmax, omega, bge, bgm, c->d_radius.as<double>(), w.eps.as<cplx>() + (size_t)z * P,
+                     c->d_mu.as<cplx>(), w.eps_sh.as<cplx>() + (size_t)z * P, c->d_kppp.as<cplx>(),
+                     c->d_kpar.as<cplx>(), c->d_gam.as<cplx>(), w.ov.as<cplx>() + z * ovs, c->d_cg.as<double>(),
+                     w.xint.as<cplx>() + z * lda, w.K.as<cplx>() + z * lda, w.K1.as<cplx>() + z * lda};
+      omx::run_shsrc(sa, st);
+    }
+    solve_all(true, w.K.as<cplx>(), w.xsh.as<cplx>());
+    for(int z = 0; z < nb; ++z)
+      omx::vmul_ov(w.xintsh.as<cplx>() + z * lda, w.xsh.as<cplx>() + z * lda, w.ov.as<cplx>() + z * ovs,
+                   omx::OV_IAUXSH1, P, nmax, w.K1.as<cplx>() + z * lda, N, st);
+  }
+  // cross sections (Result.cpp:557-761)
+  for(int z = 0; z < nb; ++z) {
+    omx::CrossArgs ca{P, nmax, w.x.as<cplx>() + z * lda, sh ? w.xsh.as<cplx>() + z * lda : nullptr,
+                      w.incloc.as<cplx>() + z * lda, w.w.as<double>() + z * ws, w.ov.as<cplx>() + z * ovs,
+                      w.cross.as<double>() + (size_t)z * 4 * P};
+    omx::run_cross(ca, st);
+  }
+  std::vector<double> part((size_t)4 * P * nb);
+  HIPCHK(hipMemcpyAsync(part.data(), w.cross.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+  auto cp = [&](double *dst, const DBuf &b) {
+    if(dst)
+      HIPCHK(hipMemcpy2DAsync(dst, sizeof(cplx) * N, b.p, sizeof(cplx) * Np, sizeof(cplx) * N, nb,
+                              hipMemcpyDeviceToHost, st));
+  };
+  cp(x_sca, w.x);
+  cp(x_int, w.xint);
+  if(sh) {
+    cp(x_sca_sh, w.xsh);
+    cp(x_int_sh, w.xintsh);
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  if(xs)
+    for(int z = 0; z < nb; ++z) {
+      double sm[4] = {0, 0, 0, 0};
+      for(int j = 0; j < P; ++j)
+        for(int e = 0; e < 4; ++e) sm[e] += part[(size_t)z * 4 * P + 4 * j + e];
+      const double rk = 2.0 * omx::kPi / lam[z] * bgn.real();
+      xs[4 * z + 0] = (-1. / (rk * rk)) * sm[0];  // Result.cpp:576
+      xs[4 * z + 1] = (1. / (rk * rk)) * sm[1];   // Result.cpp:646
+      xs[4 * z + 2] = (1. / (rk * rk)) * sm[2];   // Result.cpp:675
+      xs[4 * z + 3] = sh ? (1.0 / (4.0 * c->bg_eps_r.x * c->bg_mu_r.x)) * sm[3] : 0.0;  // Result.cpp:760
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -518,6 +946,12 @@ void omx_destroy(omx_context *c) {
                   &c->d_part, &c->d_h, &c->d_nrm, &c->d_gpart, &c->d_piv2, &c->d_perm2, &c->d_flags2,
                   &c->d_dinv, &c->d_dinv2, &c->d_F64, &c->d_F64sh};
   for(DBuf *b : bufs) b->release(&c->total);
+  auto &w = c->sw;
+  DBuf *sbufs[] = {&w.S64, &w.S32, &w.V64, &w.V32, &w.F64, &w.F64sh, &w.eps, &w.eps_sh, &w.ov, &w.incloc, &w.Q,
+                   &w.w, &w.x, &w.xint, &w.xsh, &w.xintsh, &w.K, &w.K1, &w.cross, &w.perm, &w.perm2, &w.flags,
+                   &w.flags2, &w.dinv, &w.dinv2, &w.map, &w.status, &w.basis, &w.r, &w.w1, &w.w2, &w.gpart,
+                   &w.part, &w.hd, &w.nrm, &w.sc, &w.act};
+  for(DBuf *b : sbufs) b->release(&c->total);
   for(auto &e : c->ev)
     if(e) (void)hipEventDestroy(e);
   for(auto e : c->pev)
@@ -878,6 +1312,118 @@ int omx_dense_solve(omx_context *c, int N, const double *A, const double *b, dou
     ctx.sh = saveSh;
     ctx.updated = false;
     ctx.factored = false;
+  });
+}
+
+int omx_solve_sweep(omx_context *c, int nl, const double *lambdas, const double *eps_r, const double *eps_r_sh,
+                    int batch, double *x_sca, double *x_int, double *x_sca_sh, double *x_int_sh, double *xs,
+                    int *iterations) {
+  return guard([&] {
+    if(!c || !c->have_geom || !c->have_exc || nl < 1 || !lambdas || batch < 0)
+      throw OmxError(OMX_ERR_ARG, "invalid sweep (geometry, excitation and wavelengths required)");
+    if(c->kind == OMX_SOLVER_GMRES64) throw OmxError(OMX_ERR_ARG, "sweeps need an LU solver kind");
+    for(int l = 0; l < nl; ++l)
+      if(!(lambdas[l] > 0.0)) throw OmxError(OMX_ERR_ARG, "non-positive wavelength");
+    HIPCHK(hipSetDevice(c->device));
+    alloc_all(c);
+    c->d_status.ensure(64, &c->total);
+    const int P = c->P, N = c->N;
+    // geometry and material constants on the device (as omx_update)
+    HIPCHK(hipMemcpyAsync(c->d_xyz.p, c->xyz.data(), sizeof(double) * 3 * P, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_radius.p, c->radius.data(), sizeof(double) * P, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_mu.p, c->mu.data(), sizeof(cplx) * P, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_kppp.p, c->ksippp.data(), sizeof(cplx) * P, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_kpar.p, c->ksiparppar.data(), sizeof(cplx) * P, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_gam.p, c->gamma.data(), sizeof(cplx) * P, hipMemcpyHostToDevice, c->st));
+    // batch size: at most 64 systems and half of the free device memory
+    const bool f64 = c->kind == OMX_SOLVER_LU64;
+    const double per = (double)c->Np * c->Np * ((c->sh ? 2 : 1) * (f64 ? 32.0 : 24.0));
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    int nb = batch > 0 ? batch : 64;
+    nb = std::max(1, std::min({nb, nl, (int)std::max(1.0, 0.5 * (double)(fr + c->sw.S64.bytes) / per)}));
+    std::vector<cplx> eps((size_t)nl * P), eps_sh((size_t)nl * P);
+    for(int l = 0; l < nl; ++l)
+      for(int i = 0; i < P; ++i) {
+        eps[(size_t)l * P + i] = eps_r ? kEps0 * C(eps_r, l * P + i) : c->eps[i];
+        eps_sh[(size_t)l * P + i] = eps_r_sh ? kEps0 * C(eps_r_sh, l * P + i) : c->eps_sh[i];
+      }
+    auto off = [&](double *p, int l, size_t n) { return p ? p + 2 * n * l : nullptr; };
+    std::vector<int> redo;
+    int itmax[2] = {0, 0};
+    double resmax[2] = {0, 0};
+    HIPCHK(hipEventRecord(c->ev[6], c->st));
+    for(int l0 = 0; l0 < nl; l0 += nb) {
+      const int n = std::min(nb, nl - l0);
+      std::vector<int> bad(n, 0), its(2 * n, 0);
+      std::vector<double> res(2 * n, 0.0);
+      sweep_chunk(c, n, lambdas + l0, eps.data() + (size_t)l0 * P, eps_sh.data() + (size_t)l0 * P,
+                  off(x_sca, l0, N), off(x_int, l0, N), off(x_sca_sh, l0, N), off(x_int_sh, l0, N),
+                  xs ? xs + 4 * l0 : nullptr, bad, its, res);
+      for(int z = 0; z < n; ++z) {
+        if(bad[z]) redo.push_back(l0 + z);
+        if(iterations) {
+          iterations[2 * (l0 + z)] = its[2 * z];
+          iterations[2 * (l0 + z) + 1] = its[2 * z + 1];
+        }
+        for(int e = 0; e < 2; ++e) {
+          itmax[e] = std::max(itmax[e], its[2 * z + e]);
+          if(!bad[z]) resmax[e] = std::max(resmax[e], res[2 * z + e]);
+        }
+      }
+    }
+    HIPCHK(hipEventRecord(c->ev[7], c->st));
+    HIPCHK(hipEventSynchronize(c->ev[7]));
+    const float t_batch = elapsed(c, 6, 7);
+    // systems the batch could not solve: one at a time with partial pivoting
+    if(!redo.empty()) {
+      const double lam0 = c->lambda;
+      const std::vector<cplx> e0 = c->eps, es0 = c->eps_sh;
+      const int piv0 = c->pivot;
+      c->pivot = OMX_PIVOT_PARTIAL;
+      try {
+        for(int l : redo) {
+          c->lambda = lambdas[l];
+          for(int i = 0; i < P; ++i) {
+            c->eps[i] = eps[(size_t)l * P + i];
+            c->eps_sh[i] = eps_sh[(size_t)l * P + i];
+          }
+          int rc = omx_update(c);
+          if(rc) throw OmxError(rc, omx_last_error());
+          rc = omx_solve(c, off(x_sca, l, N), off(x_int, l, N), off(x_sca_sh, l, N), off(x_int_sh, l, N));
+          if(rc) throw OmxError(rc, omx_last_error());
+          if(xs) {
+            rc = omx_cross_sections(c, xs + 4 * l);
+            if(rc) throw OmxError(rc, omx_last_error());
+          }
+          if(iterations) {
+            iterations[2 * l] = c->stats.iterations;
+            iterations[2 * l + 1] = c->stats.iterations_sh;
+          }
+          ++c->fallbacks;
+        }
+      } catch(...) {
+        c->lambda = lam0;
+        c->eps = e0;
+        c->eps_sh = es0;
+        c->pivot = piv0;
+        throw;
+      }
+      c->lambda = lam0;
+      c->eps = e0;
+      c->eps_sh = es0;
+      c->pivot = piv0;
+      c->updated = c->solved = false;
+    }
+    c->stats.N = c->N;
+    c->stats.Np = c->Np;
+    c->stats.iterations = itmax[0];
+    c->stats.iterations_sh = itmax[1];
+    c->stats.residual = resmax[0];
+    c->stats.residual_sh = resmax[1];
+    c->stats.t_solve_ms = t_batch;
+    c->stats.sweep_batch = nb;
+    c->stats.device_bytes = c->total;
   });
 }
 

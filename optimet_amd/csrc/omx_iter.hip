@@ -138,4 +138,134 @@ void nrm2sq(const cplx *x, int n, double *part, double *out, hipStream_t st) {
   hipLaunchKernelGGL(nrm2_reduce_k, dim3(1), dim3(128), 0, st, part, nb, out);
 }
 
+// ---------------------------------------------------------------- batches
+// nb systems side by side (blockIdx.y = system): vectors sv apart, Krylov
+// bases sV apart (vectors ldv apart inside a basis), per-system scalars and
+// coefficient vectors in device arrays.
+__global__ void vaxpby_b_k(cplx *y, size_t sy, const cplx *a, const cplx *x, size_t sx, const cplx *b, int n,
+                           const int *act) {
+  const int z = blockIdx.y;
+  if(act && !act[z]) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) y[z * sy + i] = a[z] * x[z * sx + i] + b[z] * y[z * sy + i];
+}
+__global__ void vsub_b_k(cplx *zz, const cplx *x, const cplx *y, int n, size_t sv) {
+  const size_t o = blockIdx.y * sv;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) zz[o + i] = x[o + i] - y[o + i];
+}
+__global__ __launch_bounds__(256) void dotc_multi_b_k(const cplx *V, size_t ldv, size_t sV, int nv, const cplx *w,
+                                                      size_t sv, int n, cplx *part) {
+  const int z = blockIdx.y;
+  V += z * sV;
+  w += z * sv;
+  part += (size_t)z * nv * gridDim.x;
+  __shared__ cplx red[256];
+  for(int k = 0; k < nv; ++k) {
+    cplx s = mk(0.0);
+    for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s = s + conj(V[k * ldv + i]) * w[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for(int st = 128; st > 0; st >>= 1) {
+      if(threadIdx.x < st) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + st];
+      __syncthreads();
+    }
+    if(threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+__global__ void reduce_rows_b_k(const cplx *part, int nb, int nv, cplx *out, int so) {
+  const int z = blockIdx.y, k = blockIdx.x;
+  part += (size_t)z * nv * nb;
+  __shared__ cplx red[128];
+  cplx s = mk(0.0);
+  for(int b = threadIdx.x; b < nb; b += 128) s = s + part[(size_t)k * nb + b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for(int st = 64; st > 0; st >>= 1) {
+    if(threadIdx.x < st) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) out[(size_t)z * so + k] = red[0];
+}
+__global__ void axpy_multi_b_k(const cplx *V, size_t ldv, size_t sV, int nv, const cplx *h, int sh, cplx *w,
+                               size_t sv, int n) {
+  const int z = blockIdx.y;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  V += z * sV;
+  h += (size_t)z * sh;
+  cplx s = w[z * sv + i];
+  for(int k = 0; k < nv; ++k) s = s - V[k * ldv + i] * h[k];
+  w[z * sv + i] = s;
+}
+__global__ void gemv_multi_b_k(const cplx *V, size_t ldv, size_t sV, int nv, const cplx *h, int sh, cplx *zz,
+                               size_t sv, int n) {
+  const int z = blockIdx.y;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  V += z * sV;
+  h += (size_t)z * sh;
+  cplx s = mk(0.0);
+  for(int k = 0; k < nv; ++k) s = s + V[k * ldv + i] * h[k];
+  zz[z * sv + i] = s;
+}
+__global__ __launch_bounds__(256) void nrm2_b_k(const cplx *x, size_t sv, int n, double *part) {
+  const int z = blockIdx.y;
+  x += z * sv;
+  __shared__ double red[256];
+  double s = 0.0;
+  for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += norm2(x[i]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for(int st = 128; st > 0; st >>= 1) {
+    if(threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) part[(size_t)z * gridDim.x + blockIdx.x] = red[0];
+}
+__global__ void nrm2_reduce_b_k(const double *part, int nb, double *out) {
+  const int z = blockIdx.x;
+  part += (size_t)z * nb;
+  __shared__ double red[128];
+  double s = 0.0;
+  for(int b = threadIdx.x; b < nb; b += 128) s += part[b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for(int st = 64; st > 0; st >>= 1) {
+    if(threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) out[z] = red[0];
+}
+
+static inline dim3 gb(int n, int nb) { return dim3((n + 255) / 256, nb); }
+void vaxpby_b(cplx *y, size_t sy, const cplx *a, const cplx *x, size_t sx, const cplx *b, int n, int nb,
+              const int *act, hipStream_t st) {
+  hipLaunchKernelGGL(vaxpby_b_k, gb(n, nb), dim3(256), 0, st, y, sy, a, x, sx, b, n, act);
+}
+void vsub_b(cplx *z, const cplx *x, const cplx *y, int n, int nb, size_t sv, hipStream_t st) {
+  hipLaunchKernelGGL(vsub_b_k, gb(n, nb), dim3(256), 0, st, z, x, y, n, sv);
+}
+int red_blocks_b(int n, int nb) { return std::max(1, std::min(kRedBlocks / std::max(1, nb / 8), (n + 255) / 256)); }
+void dotc_multi_b(const cplx *V, size_t ldv, size_t sV, int nv, const cplx *w, size_t sv, int n, int nb, cplx *part,
+                  cplx *out, int so, hipStream_t st) {
+  const int rb = red_blocks_b(n, nb);
+  hipLaunchKernelGGL(dotc_multi_b_k, dim3(rb, nb), dim3(256), 0, st, V, ldv, sV, nv, w, sv, n, part);
+  hipLaunchKernelGGL(reduce_rows_b_k, dim3(nv, nb), dim3(128), 0, st, part, rb, nv, out, so);
+}
+void axpy_multi_b(const cplx *V, size_t ldv, size_t sV, int nv, const cplx *h, int sh, cplx *w, size_t sv, int n,
+                  int nb, hipStream_t st) {
+  hipLaunchKernelGGL(axpy_multi_b_k, gb(n, nb), dim3(256), 0, st, V, ldv, sV, nv, h, sh, w, sv, n);
+}
+void gemv_multi_b(const cplx *V, size_t ldv, size_t sV, int nv, const cplx *h, int sh, cplx *z, size_t sv, int n,
+                  int nb, hipStream_t st) {
+  hipLaunchKernelGGL(gemv_multi_b_k, gb(n, nb), dim3(256), 0, st, V, ldv, sV, nv, h, sh, z, sv, n);
+}
+void nrm2sq_b(const cplx *x, size_t sv, int n, int nb, double *part, double *out, hipStream_t st) {
+  const int rb = red_blocks_b(n, nb);
+  hipLaunchKernelGGL(nrm2_b_k, dim3(rb, nb), dim3(256), 0, st, x, sv, n, part);
+  hipLaunchKernelGGL(nrm2_reduce_b_k, dim3(nb), dim3(128), 0, st, part, rb, out);
+}
+
 }  // namespace omx

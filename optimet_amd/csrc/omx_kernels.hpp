@@ -84,6 +84,19 @@ void fill_coef(double *host);
 constexpr int kNB = 64;        // LU block size (panel width, padding granule)
 constexpr int kPanelRows = 128; // rows of the panel held by one workgroup
 
+// Distances between the systems of a batch (element counts of each buffer);
+// all zero for a single system.  A batch runs with blockIdx.z = system.
+struct BatchStrides {
+  size_t a = 0;    // matrices (elements of T)
+  size_t inv = 0;  // per-panel inverse sets (elements of T)
+  size_t vec = 0;  // right-hand sides and solutions (complex FP64 elements)
+  int perm = 0;    // permutations
+  int map = 0;     // gather maps (mdst, msrc)
+  int cnt = 0;     // map counts
+  int flags = 0;   // triangular-solve flags
+  int status = 0;  // status words
+};
+
 struct LuWork {
   // device scratch of one factorization
   int *piv;             // Np: LAPACK-style swap sequence
@@ -101,6 +114,8 @@ struct LuWork {
   int gmax;
   int epoch;            // last flag value used by a triangular solve
   int pivot;            // kPivotBlock or kPivotPartial
+  int nb = 1;           // systems in the batch (block pivoting only when > 1)
+  BatchStrides bs;      // their strides; the pointers above are system 0's
 };
 // Pivot search of a panel: among the rows of its diagonal block only
 // (followed by iterative refinement), or over the whole column (tournament).
@@ -125,7 +140,8 @@ template <typename T>
 void lu_solve(const T *A, int Np, size_t lda, LuWork &w, const cplx *b, cplx *x, hipStream_t st);
 
 // ---------------------------------------------------------------- iterative
-void gemv64(const cplx *A, size_t lda, int N, const cplx *x, cplx *y, cplx *part, hipStream_t st);
+void gemv64(const cplx *A, size_t lda, int N, const cplx *x, cplx *y, cplx *part, hipStream_t st, int nb = 1,
+            size_t sA = 0, size_t sv = 0);
 constexpr int kGemvSplitMax = 32;
 
 // level-1 helpers (omx_iter.hip); reductions return through small device arrays
@@ -140,5 +156,19 @@ void dotc_multi(const cplx *V, size_t ldv, int nv, const cplx *w, int n, cplx *p
 void axpy_multi(const cplx *V, size_t ldv, int nv, const cplx *h, cplx *w, int n, hipStream_t st); // w -= V h
 void gemv_multi(const cplx *V, size_t ldv, int nv, const cplx *h, cplx *z, int n, hipStream_t st); // z = V h
 void nrm2sq(const cplx *x, int n, double *part, double *out, hipStream_t st);
+// the same for nb systems side by side (vectors sv, sy, sx apart, bases sV apart,
+// per-system scalars a[z], b[z], coefficient vectors sh apart, outputs so apart);
+// act (may be null) skips the systems whose flag is 0
+void vaxpby_b(cplx *y, size_t sy, const cplx *a, const cplx *x, size_t sx, const cplx *b, int n, int nb,
+              const int *act, hipStream_t st);
+void vsub_b(cplx *z, const cplx *x, const cplx *y, int n, int nb, size_t sv, hipStream_t st);
+int red_blocks_b(int n, int nb);  // partial sums per system of the reductions below
+void dotc_multi_b(const cplx *V, size_t ldv, size_t sV, int nv, const cplx *w, size_t sv, int n, int nb, cplx *part,
+                  cplx *out, int so, hipStream_t st);
+void axpy_multi_b(const cplx *V, size_t ldv, size_t sV, int nv, const cplx *h, int sh, cplx *w, size_t sv, int n,
+                  int nb, hipStream_t st);
+void gemv_multi_b(const cplx *V, size_t ldv, size_t sV, int nv, const cplx *h, int sh, cplx *z, size_t sv, int n,
+                  int nb, hipStream_t st);
+void nrm2sq_b(const cplx *x, size_t sv, int n, int nb, double *part, double *out, hipStream_t st);
 
 }  // namespace omx

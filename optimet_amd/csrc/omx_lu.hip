@@ -108,7 +108,8 @@ __device__ inline double rdl1(double v, int l) {
 __device__ inline float rdl1(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
 template <typename R> __device__ inline Cx<R> rdl(Cx<R> v, int l) { return Cx<R>{rdl1(v.x, l), rdl1(v.y, l)}; }
 
-__global__ void iota_kernel(int *p, int n) {
+__global__ void iota_kernel(int *p, int n, int sp) {
+  p += (size_t)blockIdx.z * sp;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i < n) p[i] = i;
 }
@@ -425,8 +426,18 @@ __device__ inline unsigned wave_max_u32(unsigned x) {
 // column-major), the block's row map (mdst <- msrc) and the permutation.
 template <typename T>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2))) void diag_lu(T *A, size_t lda, int j0, T *dinv, int *mdst, int *msrc, int *mcnt,
-                                               int *perm, int *status) {
+                                               int *perm, int *status, BatchStrides bs) {
   using C = CxOf<T>;
+  {
+    const int z = blockIdx.z;
+    A += z * bs.a;
+    dinv += z * bs.inv;
+    mdst += (size_t)z * bs.map;
+    msrc += (size_t)z * bs.map;
+    mcnt += (size_t)z * bs.cnt;
+    perm += (size_t)z * bs.perm;
+    status += (size_t)z * bs.status;
+  }
   __shared__ C colc[2][64];
   __shared__ C ipc[2];  // reciprocal of the pivot
   __shared__ int pbs[2];
@@ -558,7 +569,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
 // 16 (a small LDS footprint lets these workgroups start beside the bulk
 // update's); the block is overwritten only after all of it has been read.
 template <typename T, bool LEFT>
-__global__ __launch_bounds__(256) void blk_mul(T *A, size_t lda, int j0, int first, const T *M) {
+__global__ __launch_bounds__(256) void blk_mul(T *A, size_t lda, int j0, int first, const T *M, BatchStrides bs) {
+  A += blockIdx.z * bs.a;
+  M += blockIdx.z * bs.inv;
   using R = typename CT<T>::R;
   using V4 = typename CT<T>::V4;
   constexpr int BK = 16, PAD = 16;
@@ -647,7 +660,11 @@ __global__ __launch_bounds__(256) void blk_mul(T *A, size_t lda, int j0, int fir
 template <typename T>
 __global__ __launch_bounds__(256) void laswp_map(T *A, size_t lda, int c0, int cend, const int *__restrict__ mdst,
                                                  const int *__restrict__ msrc, const int *__restrict__ mcnt,
-                                                 int skip0) {
+                                                 int skip0, BatchStrides bs) {
+  A += blockIdx.z * bs.a;
+  mdst += (size_t)blockIdx.z * bs.map;
+  msrc += (size_t)blockIdx.z * bs.map;
+  mcnt += (size_t)blockIdx.z * bs.cnt;
   const int m = *mcnt;
   const int cl = threadIdx.x / 32, li = threadIdx.x % 32;
   int col = c0 + blockIdx.x * 8 + cl;
@@ -673,16 +690,23 @@ __global__ __launch_bounds__(256) void laswp_map(T *A, size_t lda, int c0, int c
 }
 template <typename T>
 void laswp(T *A, size_t lda, int c0, int cend, const int *mdst, const int *msrc, const int *mcnt, int skip0,
-           hipStream_t st) {
+           hipStream_t st, int nb = 1, BatchStrides bs = BatchStrides{}) {
   const int n = cend - c0 - (skip0 >= 0 ? kNB : 0);
-  if(n > 0) hipLaunchKernelGGL((laswp_map<T>), dim3((n + 7) / 8), dim3(256), 0, st, A, lda, c0, cend, mdst, msrc, mcnt, skip0);
+  if(n > 0)
+    hipLaunchKernelGGL((laswp_map<T>), dim3((n + 7) / 8, 1, nb), dim3(256), 0, st, A, lda, c0, cend, mdst, msrc, mcnt,
+                       skip0, bs);
 }
 
 // Deferred interchanges of block pivoting: panel k = blockIdx.y (>= 1)
 // applies its map (NB entries at k NB) to the columns left of it.
 template <typename T>
 __global__ __launch_bounds__(256) void laswp_left(T *A, size_t lda, const int *__restrict__ mdst,
-                                                  const int *__restrict__ msrc, const int *__restrict__ mcnt) {
+                                                  const int *__restrict__ msrc, const int *__restrict__ mcnt,
+                                                  BatchStrides bs) {
+  A += blockIdx.z * bs.a;
+  mdst += (size_t)blockIdx.z * bs.map;
+  msrc += (size_t)blockIdx.z * bs.map;
+  mcnt += (size_t)blockIdx.z * bs.cnt;
   const int k = blockIdx.y + 1;
   const int cl = threadIdx.x / 32, li = threadIdx.x % 32;
   const int col = blockIdx.x * 8 + cl;
@@ -719,7 +743,8 @@ __global__ __launch_bounds__(256) void laswp_left(T *A, size_t lda, const int *_
 // registers while the current one is multiplied.
 template <typename T, int TM, int TN, int KD, int WGM, int WGN, int BK>
 __global__ __launch_bounds__(64 * WGM * WGN) void gemm_tile(T *A, size_t lda, int row0, int col0, int k0, int M,
-                                                           int N) {
+                                                           int N, size_t sA) {
+  A += blockIdx.z * sA;
   using R = typename CT<T>::R;
   using V4 = typename CT<T>::V4;
   constexpr int NT = 64 * WGM * WGN, PAD = 16;
@@ -842,12 +867,13 @@ template <typename T> struct UpdTile {
 
 // C -= A B over an M x N block (M, N multiples of 64) with K = KD.
 template <typename T, int KD>
-void trailing_update(T *A, size_t lda, int row0, int col0, int k0, int M, int N, hipStream_t st) {
+void trailing_update(T *A, size_t lda, int row0, int col0, int k0, int M, int N, hipStream_t st, int nb = 1,
+                     size_t sA = 0) {
   if(M <= 0 || N <= 0) return;
   using U = UpdTile<T>;
   hipLaunchKernelGGL((gemm_tile<T, U::TM, U::TN, KD, U::WGM, U::WGN, U::BK>),
-                     dim3((M + U::TM - 1) / U::TM, (N + U::TN - 1) / U::TN), dim3(64 * U::WGM * U::WGN), 0, st, A, lda,
-                     row0, col0, k0, M, N);
+                     dim3((M + U::TM - 1) / U::TM, (N + U::TN - 1) / U::TN, nb), dim3(64 * U::WGM * U::WGN), 0, st, A,
+                     lda, row0, col0, k0, M, N, sA);
 }
 
 // ------------------------------------------------------------------ driver
@@ -871,11 +897,14 @@ template <typename T>
 int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof) {
   if(Np % kNB) return -1;
   const bool block = w.pivot == kPivotBlock;
+  const int nb = w.nb;
+  const BatchStrides bs = w.bs;
+  if(!block && nb != 1) return -3;  // the tournament factors one system at a time
   hipStream_t sb = w.aux;
   T *wlu = reinterpret_cast<T *>(w.wlu);
   T *dinv = reinterpret_cast<T *>(w.dinv);
   auto dk = [&](int j0) { return dinv + (size_t)(j0 / kNB) * 2 * kNB * kNB; };
-  hipLaunchKernelGGL(iota_kernel, dim3((Np + 255) / 256), dim3(256), 0, st, w.perm, Np);
+  hipLaunchKernelGGL(iota_kernel, dim3((Np + 255) / 256, 1, nb), dim3(256), 0, st, w.perm, Np, bs.perm);
   if(prof) {
     prof->used = 0;
     prof->gemm_flop = 0.0;
@@ -901,38 +930,41 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
       map(p1, &d1, &s1, &c1);
       (void)hipStreamWaitEvent(sb, w.ev_next, 0);
       if(rec) (void)hipEventRecord(pe[0], sb);
-      hipLaunchKernelGGL((diag_lu<T>), dim3(1), dim3(512), 0, sb, A, lda, p0, dk(p0), d0, s0, c0, w.perm, w.status);
+      hipLaunchKernelGGL((diag_lu<T>), dim3(1, 1, nb), dim3(512), 0, sb, A, lda, p0, dk(p0), d0, s0, c0, w.perm,
+                         w.status, bs);
       if(rest0 > 0)
-        hipLaunchKernelGGL((blk_mul<T, false>), dim3(rest0 / 64), dim3(256), 0, sb, A, lda, p0, p1, dk(p0) + kNB * kNB);
+        hipLaunchKernelGGL((blk_mul<T, false>), dim3(rest0 / 64, 1, nb), dim3(256), 0, sb, A, lda, p0, p1,
+                           dk(p0) + kNB * kNB, bs);
       if(two) {
-        laswp<T>(A, lda, p1, p1 + kNB, d0, s0, c0, -1, sb);
-        hipLaunchKernelGGL((blk_mul<T, true>), dim3(1), dim3(256), 0, sb, A, lda, p0, p1, dk(p0));
-        trailing_update<T, kNB>(A, lda, p1, p1, p0, Np - p1, kNB, sb);
-        hipLaunchKernelGGL((diag_lu<T>), dim3(1), dim3(512), 0, sb, A, lda, p1, dk(p1), d1, s1, c1, w.perm, w.status);
+        laswp<T>(A, lda, p1, p1 + kNB, d0, s0, c0, -1, sb, nb, bs);
+        hipLaunchKernelGGL((blk_mul<T, true>), dim3(1, 1, nb), dim3(256), 0, sb, A, lda, p0, p1, dk(p0), bs);
+        trailing_update<T, kNB>(A, lda, p1, p1, p0, Np - p1, kNB, sb, nb, bs.a);
+        hipLaunchKernelGGL((diag_lu<T>), dim3(1, 1, nb), dim3(512), 0, sb, A, lda, p1, dk(p1), d1, s1, c1, w.perm,
+                           w.status, bs);
         if(restf > 0)
-          hipLaunchKernelGGL((blk_mul<T, false>), dim3(restf / 64), dim3(256), 0, sb, A, lda, p1, far,
-                             dk(p1) + kNB * kNB);
+          hipLaunchKernelGGL((blk_mul<T, false>), dim3(restf / 64, 1, nb), dim3(256), 0, sb, A, lda, p1, far,
+                             dk(p1) + kNB * kNB, bs);
       }
       if(rec) (void)hipEventRecord(pe[1], sb);
       if(two && restf > 0) {
         (void)hipStreamWaitEvent(sb, w.ev_bulk, 0);
-        laswp<T>(A, lda, far, Np, d0, s0, c0, -1, sb);
-        hipLaunchKernelGGL((blk_mul<T, true>), dim3(restf / 64), dim3(256), 0, sb, A, lda, p0, far, dk(p0));
-        trailing_update<T, kNB>(A, lda, p1, far, p0, kNB, restf, sb);
-        laswp<T>(A, lda, far, Np, d1, s1, c1, -1, sb);
-        hipLaunchKernelGGL((blk_mul<T, true>), dim3(restf / 64), dim3(256), 0, sb, A, lda, p1, far, dk(p1));
+        laswp<T>(A, lda, far, Np, d0, s0, c0, -1, sb, nb, bs);
+        hipLaunchKernelGGL((blk_mul<T, true>), dim3(restf / 64, 1, nb), dim3(256), 0, sb, A, lda, p0, far, dk(p0), bs);
+        trailing_update<T, kNB>(A, lda, p1, far, p0, kNB, restf, sb, nb, bs.a);
+        laswp<T>(A, lda, far, Np, d1, s1, c1, -1, sb, nb, bs);
+        hipLaunchKernelGGL((blk_mul<T, true>), dim3(restf / 64, 1, nb), dim3(256), 0, sb, A, lda, p1, far, dk(p1), bs);
       }
       (void)hipEventRecord(w.ev_pan, sb);
       (void)hipStreamWaitEvent(st, w.ev_pan, 0);
       if(rec) (void)hipEventRecord(pe[2], st);
       if(two && restf > 0) {
         const int nn = restf < 2 * kNB ? restf : 2 * kNB;
-        trailing_update<T, 2 * kNB>(A, lda, far, far, p0, restf, nn, st);
+        trailing_update<T, 2 * kNB>(A, lda, far, far, p0, restf, nn, st, nb, bs.a);
         (void)hipEventRecord(w.ev_next, st);
-        trailing_update<T, 2 * kNB>(A, lda, far, far + nn, p0, restf, restf - nn, st);
+        trailing_update<T, 2 * kNB>(A, lda, far, far + nn, p0, restf, restf - nn, st, nb, bs.a);
         (void)hipEventRecord(w.ev_bulk, st);
         if(rec) {
-          prof->gemm_flop += 8.0 * (double)restf * restf * (2 * kNB);
+          prof->gemm_flop += 8.0 * (double)restf * restf * (2 * kNB) * nb;
           prof->gemm_launches += restf > nn ? 2 : 1;
         }
       }
@@ -942,8 +974,8 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
       }
     }
     if(Np > kNB)
-      hipLaunchKernelGGL((laswp_left<T>), dim3((Np - kNB) / 8, Np / kNB - 1), dim3(256), 0, st, A, lda, w.mdst, w.msrc,
-                         w.mcnt);
+      hipLaunchKernelGGL((laswp_left<T>), dim3((Np - kNB) / 8, Np / kNB - 1, nb), dim3(256), 0, st, A, lda, w.mdst,
+                         w.msrc, w.mcnt, bs);
     return 0;
   }
 
@@ -975,9 +1007,10 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
   auto solves = [&](int j0) {
     const int rest = Np - j0 - kNB;
     if(rest > 0) {
-      hipLaunchKernelGGL((blk_mul<T, true>), dim3(rest / 64), dim3(256), 0, st, A, lda, j0, j0 + kNB, dk(j0));
+      hipLaunchKernelGGL((blk_mul<T, true>), dim3(rest / 64), dim3(256), 0, st, A, lda, j0, j0 + kNB, dk(j0),
+                         BatchStrides{});
       hipLaunchKernelGGL((blk_mul<T, false>), dim3(rest / 64), dim3(256), 0, st, A, lda, j0, j0 + kNB,
-                         dk(j0) + kNB * kNB);
+                         dk(j0) + kNB * kNB, BatchStrides{});
     }
   };
   if((Np + kPanelRows - 1) / kPanelRows > w.gmax) return -2;
@@ -1022,7 +1055,10 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
 }
 
 // ------------------------------------------------------------------ solves
-__global__ void gather_kernel(const cplx *b, const int *perm, cplx *y, int Np) {
+__global__ void gather_kernel(const cplx *b, const int *perm, cplx *y, int Np, BatchStrides bs) {
+  b += blockIdx.z * bs.vec;
+  y += blockIdx.z * bs.vec;
+  perm += (size_t)blockIdx.z * bs.perm;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i < Np) y[i] = b[perm[i]];
 }
@@ -1042,9 +1078,13 @@ __global__ void gather_kernel(const cplx *b, const int *perm, cplx *y, int Np) {
 template <typename T, bool LOWER, int SB, int PRE>
 __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size_t lda, int Np,
                                                    const T *__restrict__ dinv, cplx *x, int *flags, int epoch,
-                                                   int *status) {
+                                                   int *status, BatchStrides bs) {
   constexpr int NB = kNB;
-  constexpr int NI = SB * (SB - 1) / 2;  // intra-super-block tiles
+  constexpr int NI = SB * (SB - 1) / 2;
+  A += blockIdx.z * bs.a;
+  dinv += blockIdx.z * bs.inv;
+  x += blockIdx.z * bs.vec;
+  flags += (size_t)blockIdx.z * bs.flags;  // intra-super-block tiles
   const int nblk = Np / NB, nsb = (nblk + SB - 1) / SB;
   const int K = LOWER ? blockIdx.x : nsb - 1 - blockIdx.x;
   const int kb0 = K * SB, nb = min(SB, nblk - kb0);
@@ -1239,14 +1279,14 @@ void lu_solve(const T *A, int Np, size_t lda, LuWork &w, const cplx *b, cplx *x,
   const T *dinv = reinterpret_cast<const T *>(w.dinv);
   constexpr int SB = TrsvCfg<T>::SB;
   constexpr int PRE = TrsvCfg<T>::PRE;
-  hipLaunchKernelGGL(gather_kernel, dim3((Np + 255) / 256), dim3(256), 0, st, b, w.perm, x, Np);
+  hipLaunchKernelGGL(gather_kernel, dim3((Np + 255) / 256, 1, w.nb), dim3(256), 0, st, b, w.perm, x, Np, w.bs);
   const int nblk = Np / kNB, nsb = (nblk + SB - 1) / SB;
   int e1 = ++w.epoch;
-  hipLaunchKernelGGL((trsv_kernel<T, true, SB, PRE>), dim3(nsb), dim3(256), 0, st, A, lda, Np, dinv, x, w.flags, e1,
-                     w.status);
+  hipLaunchKernelGGL((trsv_kernel<T, true, SB, PRE>), dim3(nsb, 1, w.nb), dim3(256), 0, st, A, lda, Np, dinv, x,
+                     w.flags, e1, w.status, w.bs);
   int e2 = ++w.epoch;
-  hipLaunchKernelGGL((trsv_kernel<T, false, SB, PRE>), dim3(nsb), dim3(256), 0, st, A, lda, Np, dinv, x, w.flags, e2,
-                     w.status);
+  hipLaunchKernelGGL((trsv_kernel<T, false, SB, PRE>), dim3(nsb, 1, w.nb), dim3(256), 0, st, A, lda, Np, dinv, x,
+                     w.flags, e2, w.status, w.bs);
 }
 
 template int lu_factor<double2>(double2 *, int, size_t, LuWork &, hipStream_t, LuProf *);
@@ -1258,7 +1298,11 @@ template void lu_solve<float2>(const float2 *, int, size_t, LuWork &, const cplx
 // y = A x for the FP64 matrix (residuals and GMRES), two-stage column split.
 constexpr int kGemvSplit = 32;
 __global__ __launch_bounds__(256) void gemv_part_kernel(const cplx *__restrict__ A, size_t lda, int N,
-                                                        const cplx *__restrict__ x, cplx *part, int chunk) {
+                                                        const cplx *__restrict__ x, cplx *part, int chunk, size_t sA,
+                                                        size_t sv) {
+  A += blockIdx.z * sA;
+  x += blockIdx.z * sv;
+  part += (size_t)blockIdx.z * gridDim.y * N;
   __shared__ cplx xs[512];
   const int r = blockIdx.x * 256 + threadIdx.x;
   const int c0 = blockIdx.y * chunk, c1 = min(N, c0 + chunk);
@@ -1281,18 +1325,23 @@ __global__ __launch_bounds__(256) void gemv_part_kernel(const cplx *__restrict__
   }
   if(r < N) part[(size_t)blockIdx.y * N + r] = acc;
 }
-__global__ void gemv_reduce_kernel(const cplx *part, int ns, int N, cplx *y) {
+__global__ void gemv_reduce_kernel(const cplx *part, int ns, int N, cplx *y, size_t sv) {
+  part += (size_t)blockIdx.z * ns * N;
+  y += blockIdx.z * sv;
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if(r >= N) return;
   cplx s = mk(0.0);
   for(int k = 0; k < ns; ++k) s = s + part[(size_t)k * N + r];
   y[r] = s;
 }
-void gemv64(const cplx *A, size_t lda, int N, const cplx *x, cplx *y, cplx *part, hipStream_t st) {
+// y = A x for nb systems (matrices sA, vectors sv apart); part holds
+// kGemvSplit N partial vectors per system
+void gemv64(const cplx *A, size_t lda, int N, const cplx *x, cplx *y, cplx *part, hipStream_t st, int nb, size_t sA,
+            size_t sv) {
   int chunk = (N + kGemvSplit - 1) / kGemvSplit;
-  hipLaunchKernelGGL(gemv_part_kernel, dim3((N + 255) / 256, kGemvSplit), dim3(256), 0, st, A, lda, N, x, part,
-                     chunk);
-  hipLaunchKernelGGL(gemv_reduce_kernel, dim3((N + 255) / 256), dim3(256), 0, st, part, kGemvSplit, N, y);
+  hipLaunchKernelGGL(gemv_part_kernel, dim3((N + 255) / 256, kGemvSplit, nb), dim3(256), 0, st, A, lda, N, x, part,
+                     chunk, sA, sv);
+  hipLaunchKernelGGL(gemv_reduce_kernel, dim3((N + 255) / 256, 1, nb), dim3(256), 0, st, part, kGemvSplit, N, y, sv);
 }
 
 }  // namespace omx

@@ -26,7 +26,7 @@ ERRORS = {-1: "argument", -2: "device", -3: "memory", -4: "singular", -5: "state
 EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "omx_set_excitation",
            "omx_set_solver", "omx_set_cg_tables", "omx_build_cg_tables", "omx_update", "omx_solve",
            "omx_cross_sections", "omx_scattering_size", "omx_get_stats", "omx_read_matrix", "omx_read_vector",
-           "omx_dense_solve", "omx_set_profile", "omx_set_pivoting"]
+           "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_solve_sweep"]
 
 
 class OmxError(RuntimeError):
@@ -43,7 +43,7 @@ class Stats(ctypes.Structure):
                 ("device_bytes", ctypes.c_size_t), ("t_lu_gemm_ms", ctypes.c_double),
                 ("lu_gemm_flop", ctypes.c_double), ("lu_gemm_launches", ctypes.c_int),
                 ("t_lu_panel_ms", ctypes.c_double), ("t_factor_total_ms", ctypes.c_double),
-                ("pivot_fallbacks", ctypes.c_int)]
+                ("pivot_fallbacks", ctypes.c_int), ("sweep_batch", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -80,6 +80,7 @@ def load_library(path=LIB_PATH):
     L.omx_dense_solve.argtypes = [P, I, D, D, D, I]
     L.omx_set_profile.argtypes = [P, I]
     L.omx_set_pivoting.argtypes = [P, I]
+    L.omx_solve_sweep.argtypes = [P, I, D, D, D, I, D, D, D, D, D, ctypes.POINTER(ctypes.c_int)]
     _lib = L
     return L
 
@@ -190,6 +191,36 @@ class Solver:
         out = {"x_sca": xs, "x_int": xi}
         if self.cluster.sh:
             out.update({"x_sca_sh": xss, "x_int_sh": xis})
+        return out
+
+    def sweep(self, wavelengths, batch=0, outputs=("x_sca", "x_int", "x_sca_sh", "x_int_sh")):
+        """Batched wavelength sweep (omx_solve_sweep): the cluster at every
+        wavelength, Drude spheres re-evaluated per wavelength.  Returns a dict
+        with the requested (n_lambda, N) coefficient arrays, "xs" (n_lambda, 4:
+        C_ext, C_sca, C_abs, C_sca_SH) and "iterations" (n_lambda, 2)."""
+        c = self.cluster
+        lams = np.ascontiguousarray(np.asarray(wavelengths, dtype=np.float64))
+        nl, N = lams.size, c.N
+        eps = np.empty((nl, c.nobj), np.complex128)
+        eps_sh = np.empty((nl, c.nobj), np.complex128)
+        for i, lam in enumerate(lams):
+            ci = c.with_wavelength(lam)
+            eps[i], eps_sh[i] = ci.eps_r, ci.eps_r_sh
+        out = {}
+        ptrs = []
+        for name in ("x_sca", "x_int", "x_sca_sh", "x_int_sh"):
+            if name in outputs and (c.sh or not name.endswith("_sh")):
+                out[name] = np.zeros((nl, N), np.complex128)
+                ptrs.append(_d(out[name].view(np.float64)))
+            else:
+                ptrs.append(None)
+        xs = np.zeros((nl, 4))
+        its = np.zeros((nl, 2), np.int32)
+        _check(self.ctx.L.omx_solve_sweep(self.ctx.h, nl, _d(lams), _d(eps.view(np.float64)),
+                                          _d(eps_sh.view(np.float64)), int(batch), *ptrs, _d(xs),
+                                          its.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        out["xs"] = xs
+        out["iterations"] = its
         return out
 
     def cross_sections(self):

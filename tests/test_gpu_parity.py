@@ -213,3 +213,41 @@ def test_golden_fixtures_gpu(ctx, kind, tol):
         if "x_sca_sh" in ref.files:
             assert rel(out["x_sca_sh"], ref["x_sca_sh"]) < 100 * tol, name
             assert abs(cs["csca_sh"] - ref["csca_sh"]) <= 100 * tol * abs(ref["csca_sh"]), name
+
+
+@pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL)])
+@pytest.mark.parametrize("P,nmax,batch", [(5, 4, 3), (8, 6, 4)])
+def test_wavelength_sweep_batched(ctx, ora, kind, tol, P, nmax, batch):
+    """omx_solve_sweep: several wavelengths solved as batches (chunks of
+    `batch` systems, the last one partial) against the oracle at each
+    wavelength, Drude permittivities re-evaluated per wavelength."""
+    from optimet_amd import solver
+    c = drude_cluster(P, nmax, 600e-9, 40, 80, 600 if P < 8 else 800, 10, seed=31 + P)
+    lams = np.linspace(450e-9, 850e-9, 7)
+    s = solver.Solver(c, kind=kind, context=ctx)
+    out = s.sweep(lams, batch=batch)
+    assert s.stats()["sweep_batch"] == batch
+    for i, lam in enumerate(lams):
+        ref = ora.solve_all(c.with_wavelength(lam))
+        assert rel(out["x_sca"][i], ref["x_sca"]) < tol, (i, lam)
+        assert rel(out["x_sca_sh"][i], ref["x_sca_sh"]) < 100 * tol, (i, lam)
+        xs = out["xs"][i]
+        for j, key in enumerate(("cext", "csca", "cabs", "csca_sh")):
+            t = tol if j < 3 else 100 * tol
+            assert abs(xs[j] - ref[key]) <= t * abs(ref[key]), (i, key)
+        assert out["iterations"][i, 0] > 0
+
+
+def test_sweep_matches_single_solves(ctx):
+    """A batched sweep and the same wavelengths solved one at a time agree."""
+    from optimet_amd import solver
+    c = drude_cluster(6, 5, 700e-9, 40, 80, 700, 10, seed=77)
+    lams = np.array([520e-9, 610e-9, 760e-9])
+    s = solver.Solver(c, kind="lu32ir", context=ctx)
+    out = s.sweep(lams, batch=0)
+    for i, lam in enumerate(lams):
+        s1 = solver.Solver(c.with_wavelength(lam), kind="lu32ir", context=ctx)
+        s1.update()
+        one = s1.solve()
+        assert rel(out["x_sca"][i], one["x_sca"]) < 1e-12
+        assert rel(out["x_sca_sh"][i], one["x_sca_sh"]) < 1e-10
