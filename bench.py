@@ -93,6 +93,161 @@ def run_path(cluster, kind, steps, warmup, profile):
     return {"lat": lat, "total": total, "out": out, "cs": cs, "stats": st}
 
 
+def run_sweep(cluster, lams, kind, steps, warmup, batch):
+    """Batched wavelength sweeps (omx_solve_sweep): every step solves all
+    wavelengths at omega and 2 omega."""
+    from optimet_amd import solver
+
+    ctx = solver.Context()
+    s = solver.Solver(cluster, kind=kind, context=ctx)
+    for _ in range(warmup):
+        s.sweep(lams, batch=batch, outputs=())
+    lat = []
+    out = None
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        out = s.sweep(lams, batch=batch, outputs=("x_sca", "x_sca_sh"))
+        lat.append(time.perf_counter() - t0)
+    total = time.perf_counter() - t_start
+    st = s.stats()
+    ctx.close()
+    return {"lat": lat, "total": total, "out": out, "stats": st}
+
+
+def sweep_main(args):
+    """Config 3: one 16-sphere Drude cluster, nmax 10 (N = 3840), 256
+    wavelengths in [400, 900] nm, omega and 2 omega, batched."""
+    from optimet_amd.problem import config3_cluster, config3_wavelengths
+
+    c = config3_cluster()
+    lams = config3_wavelengths(256)
+    mixed = run_sweep(c, lams, "lu32ir", args.steps, args.warmup, args.batch)
+    fp64 = None if args.no_fp64 else run_sweep(c, lams, "lu64", args.steps, args.warmup, args.batch)
+    nl = lams.size
+    value = args.steps * nl / mixed["total"]
+    ms = mixed["stats"]
+    line = {
+        "metric": "cluster solves/sec + p99 latency, mixed-precision vs FP64 LU at rel err <=1e-9",
+        "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * mixed["total"] / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 LU (MFMA) + c128 GMRES-IR", "data": "synthetic (seeded Drude cluster)",
+        "config": {"workload": c.meta["name"], "P": c.nobj, "nmax": c.nmax, "N": c.N, "wavelengths": nl,
+                   "frequencies": "omega+2omega", "solver": "lu32ir", "batch": ms["sweep_batch"]},
+        "step": "one sweep of all wavelengths (a solve = one wavelength at omega and 2 omega)",
+        "p50_ms_per_sweep": 1e3 * pct(mixed["lat"], 50), "p99_ms_per_sweep": 1e3 * pct(mixed["lat"], 99),
+        "max_iterations": [ms["iterations"], ms["iterations_sh"]], "max_residual": [ms["residual"], ms["residual_sh"]],
+        "roofline": None,
+    }
+    if fp64 is not None:
+        fs = fp64["stats"]
+        rel = lambda a, b: float(np.max(np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)))
+        line["fp64_lu"] = {"solves_per_s": args.steps * nl / fp64["total"],
+                           "p50_ms_per_sweep": 1e3 * pct(fp64["lat"], 50),
+                           "max_iterations": [fs["iterations"], fs["iterations_sh"]]}
+        line["speedup_vs_fp64"] = value / line["fp64_lu"]["solves_per_s"]
+        mo, fo = mixed["out"], fp64["out"]
+        line["max_rel_err_vs_fp64"] = {"x_sca": rel(mo["x_sca"], fo["x_sca"]),
+                                       "x_sca_sh": rel(mo["x_sca_sh"], fo["x_sca_sh"]),
+                                       "cext": float(np.max(np.abs(mo["xs"][:, 0] - fo["xs"][:, 0])
+                                                            / np.abs(fo["xs"][:, 0])))}
+    print(json.dumps(line), flush=True)
+
+
+def config4_main(args):
+    """Config 4: 512 Drude spheres, nmax 6 (N = 49152, a 38.7 GB FP64 matrix),
+    fundamental frequency: unpreconditioned FP64 GMRES (HBM-bound matvec)
+    against the mixed-precision LU + GMRES-IR."""
+    from optimet_amd import solver
+    from optimet_amd.problem import config4
+
+    c = config4()
+    res = {}
+    for kind in ("gmres64", "lu32ir"):
+        ctx = solver.Context()
+        s = solver.Solver(c, kind=kind, context=ctx, tol=1e-10 if kind == "gmres64" else 1e-16, maxit=800)
+        ctx.set_profile(True)
+        t0 = time.perf_counter()
+        s.update()
+        t1 = time.perf_counter()
+        converged = True
+        try:
+            out = s.solve()
+            cs = s.cross_sections()
+        except solver.OmxError as e:  # the unpreconditioned GMRES may stop above its tolerance
+            if e.code != -6:
+                raise
+            converged, out, cs = False, None, {"cext": None}
+        t2 = time.perf_counter()
+        st = s.stats()
+        ctx.close()
+        bytes_mv = 16.0 * st["Np"] * st["Np"]
+        res[kind] = {"solve_s": t2 - t1, "update_s": t1 - t0, "converged": converged, "iterations": st["iterations"],
+                     "residual": st["residual"], "matvec_calls": st["matvec_calls"],
+                     "matvec_GBs": (st["matvec_calls"] * bytes_mv / (st["t_matvec_ms"] * 1e-3) / 1e9)
+                     if st["t_matvec_ms"] > 0 else None,
+                     "out": out, "cext": cs["cext"], "stats": st}
+        if kind == "lu32ir":
+            roof = roofline(st, "lu32ir", FP32_MFMA_TFLOPS)
+            res[kind]["roofline"] = roof
+    g, m = res["gmres64"], res["lu32ir"]
+    rel = (float(np.linalg.norm(m["out"]["x_sca"] - g["out"]["x_sca"]) / np.linalg.norm(g["out"]["x_sca"]))
+           if g["out"] is not None and m["out"] is not None else None)
+    line = {
+        "metric": "cluster solves/sec + p99 latency, mixed-precision vs FP64 LU at rel err <=1e-9",
+        "value": 1.0 / (m["update_s"] + m["solve_s"]), "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1e3 * (m["update_s"] + m["solve_s"]), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 LU (MFMA) + c128 GMRES-IR", "data": "synthetic (seeded Drude cluster)",
+        "config": {"workload": c.meta["name"], "P": c.nobj, "nmax": c.nmax, "N": c.N, "frequencies": "omega",
+                   "solver": "lu32ir"},
+        "roofline": m.get("roofline"),
+        "mixed_lu": {k: m[k] for k in ("update_s", "solve_s", "converged", "iterations", "residual", "matvec_calls",
+                                        "matvec_GBs", "cext")},
+        "gmres64": {k: g[k] for k in ("update_s", "solve_s", "converged", "iterations", "residual", "matvec_calls",
+                                       "matvec_GBs", "cext")},
+        "rel_diff_x_sca_mixed_vs_gmres": rel,
+        "device_GB": m["stats"]["device_bytes"] / 1e9,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def config5_main(args):
+    """Config 5: the seeded Poisson request stream (P ~ U{2..128}, nmax ~
+    U{4..10}, 1-64 wavelengths, Drude or constant permittivity, omega and
+    2 omega) served in real time by the request loop; a bounded sample: the
+    first `--requests` requests, admission closed after `--deadline` s."""
+    from optimet_amd.problem import serving_requests
+    from optimet_amd.serving import Request, Server, summarize
+
+    stream = serving_requests(n=args.requests)
+    reqs = [Request(r["id"], r["arrival"], r["cluster"], r["wavelengths"]) for r in stream]
+    nl = {r.id: r.wavelengths.size for r in reqs}
+    out = {}
+    for kind in (("lu32ir",) if args.no_fp64 else ("lu32ir", "lu64")):
+        srv = Server(slots=args.slots, kind=kind)
+        try:
+            res = srv.run(reqs, realtime=True, deadline=args.deadline)
+        finally:
+            srv.close()
+        out[kind] = (summarize(res, nl), res)
+    m = out["lu32ir"][0]
+    line = {
+        "metric": "cluster solves/sec + p99 latency, mixed-precision vs FP64 LU at rel err <=1e-9",
+        "value": m["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1e3 * m["span_s"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 LU (MFMA) + c128 GMRES-IR", "data": "synthetic (seeded request stream)",
+        "config": {"workload": "config5-serving-stream", "requests": args.requests, "slots": args.slots,
+                   "deadline_s": args.deadline, "rate_hz": 50.0, "solver": "lu32ir"},
+        "step": "the served sample; a solve = one wavelength of a request at omega and 2 omega",
+        "p50_latency_s": m["p50_s"], "p99_latency_s": m["p99_s"], "mixed": m, "roofline": None,
+    }
+    if "lu64" in out:
+        f = out["lu64"][0]
+        line["fp64_lu"] = f
+        line["speedup_vs_fp64"] = (m["solves_per_s"] / f["solves_per_s"]) if f["solves_per_s"] else None
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(cluster):
     """The oracle (C++ FP64 restatement, 1 core) on a bounded sample of the
     same workload, extrapolated per solve: pair blocks O(P^2), dense QR solves
@@ -116,6 +271,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 LU comparison path")
+    ap.add_argument("--workload", choices=("config2", "config3", "config4", "config5"), default="config2",
+                    help="config2: one 64-sphere cluster (headline); config3: 256-wavelength batched sweep; "
+                         "config4: 512 spheres, N = 49152, GMRES vs mixed LU; config5: request stream")
+    ap.add_argument("--requests", type=int, default=16, help="config5: requests of the stream served")
+    ap.add_argument("--slots", type=int, default=3, help="config5: concurrent requests")
+    ap.add_argument("--deadline", type=float, default=40.0, help="config5: admission closes after this many s")
+    ap.add_argument("--batch", type=int, default=0, help="systems per batch of the config3 sweep (0: automatic)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer (calibration of rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -134,6 +296,16 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this single-GPU build "
               "(the multi-GPU layout is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "config3":
+        sweep_main(args)
+        return
+    if args.workload == "config4":
+        config4_main(args)
+        return
+    if args.workload == "config5":
+        config5_main(args)
+        return
 
     from optimet_amd.problem import config2
 

@@ -82,6 +82,8 @@ typedef struct {
   double t_factor_total_ms; /* whole factorisations */
   int pivot_fallbacks;   /* refactorisations with partial pivoting since creation */
   int sweep_batch;       /* systems per batch of the last omx_solve_sweep */
+  int matvec_calls;      /* FP64 matrix-vector products of the iterative solvers (since omx_set_profile) */
+  double t_matvec_ms;    /* their HIP-event time (profiling on) */
 } omx_stats;
 
 const char *omx_last_error(void);
@@ -169,6 +171,13 @@ int omx_read_vector(omx_context *ctx, int which, double *out);
 int omx_solve_sweep(omx_context *ctx, int nl, const double *lambdas, const double *eps_r, const double *eps_r_sh,
                     int batch, double *x_sca, double *x_int, double *x_sca_sh, double *x_int_sh, double *xs,
                     int *iterations);
+
+/* Device memory: free and total bytes of the context's device, bytes the
+ * context holds (the footprint a scheduler admits requests against). */
+int omx_mem_info(omx_context *ctx, size_t *free_bytes, size_t *total_bytes, size_t *held_bytes);
+/* Releases the context's matrix-sized buffers (matrices, factors, Krylov
+ * bases); the next update or sweep allocates them again. */
+int omx_trim(omx_context *ctx);
 
 /* Stand-alone dense complex solve A x = b on the device (A column-major,
  * N x N); kind OMX_SOLVER_LU64 or OMX_SOLVER_LU32_IR.  Uses the context's

@@ -111,6 +111,8 @@ struct omx_context {
     DBuf perm, perm2, flags, flags2, dinv, dinv2, map, status, basis, r, w1, w2, gpart, part, hd, nrm, sc, act;
   } sw;
   omx_stats stats{};
+  long long mv_calls = 0;  // FP64 matrix-vector products since the last omx_set_profile
+  double mv_ms = 0;
   double omega() const { return omx::kC0 * (2.0 * omx::kPi / lambda); }
   cplx waveK() const {
     std::complex<double> b = std::sqrt(std::complex<double>(bg_eps_r.x, bg_eps_r.y) *
@@ -258,7 +260,18 @@ struct Gmres {
     HIPCHK(hipStreamSynchronize(c->st));
     return std::sqrt(v);
   }
-  void matvec(const cplx *x, cplx *y) { omx::gemv64(A, (size_t)c->Np, N, x, y, c->d_gpart.as<cplx>(), c->st); }
+  void matvec(const cplx *x, cplx *y) {
+    if(c->profile) HIPCHK(hipEventRecord(c->ev[6], c->st));
+    omx::gemv64(A, (size_t)c->Np, N, x, y, c->d_gpart.as<cplx>(), c->st);
+    ++c->mv_calls;
+    if(c->profile) {
+      HIPCHK(hipEventRecord(c->ev[7], c->st));
+      HIPCHK(hipEventSynchronize(c->ev[7]));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
+      c->mv_ms += ms;
+    }
+  }
   // Solves A x = b; x holds the initial guess.  Returns iterations; *res the
   // final true relative residual.
   int solve(const cplx *b, cplx *x, double tol, int maxit, int m, double *res) {
@@ -1225,6 +1238,8 @@ int omx_get_stats(const omx_context *c, omx_stats *out) {
   out->t_lu_panel_ms = c->prof_panel_ms;
   out->t_factor_total_ms = c->prof_total_ms;
   out->pivot_fallbacks = c->fallbacks;
+  out->matvec_calls = (int)c->mv_calls;
+  out->t_matvec_ms = c->mv_ms;
   return OMX_OK;
 }
 
@@ -1234,6 +1249,8 @@ int omx_set_profile(omx_context *c, int on) {
     c->profile = on != 0;
     c->prof_gemm_ms = c->prof_panel_ms = c->prof_flop = c->prof_total_ms = 0;
     c->prof_launches = 0;
+    c->mv_calls = 0;
+    c->mv_ms = 0;
   });
 }
 
@@ -1424,6 +1441,32 @@ int omx_solve_sweep(omx_context *c, int nl, const double *lambdas, const double 
     c->stats.t_solve_ms = t_batch;
     c->stats.sweep_batch = nb;
     c->stats.device_bytes = c->total;
+  });
+}
+
+int omx_mem_info(omx_context *c, size_t *free_bytes, size_t *total_bytes, size_t *held_bytes) {
+  return guard([&] {
+    if(!c) throw OmxError(OMX_ERR_ARG, "null context");
+    HIPCHK(hipSetDevice(c->device));
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    if(free_bytes) *free_bytes = fr;
+    if(total_bytes) *total_bytes = tot;
+    if(held_bytes) *held_bytes = c->total;
+  });
+}
+
+int omx_trim(omx_context *c) {
+  return guard([&] {
+    if(!c) throw OmxError(OMX_ERR_ARG, "null context");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->st));
+    auto &w = c->sw;
+    DBuf *big[] = {&c->d_S64, &c->d_S32, &c->d_V64, &c->d_V32, &c->d_F64, &c->d_F64sh, &w.S64, &w.S32, &w.V64,
+                   &w.V32, &w.F64, &w.F64sh, &w.basis, &w.gpart, &c->d_basis, &c->d_gpart};
+    for(DBuf *b : big) b->release(&c->total);
+    c->updated = c->solved = false;
+    c->factored = c->factored_sh = false;
   });
 }
 

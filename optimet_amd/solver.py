@@ -26,7 +26,7 @@ ERRORS = {-1: "argument", -2: "device", -3: "memory", -4: "singular", -5: "state
 EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "omx_set_excitation",
            "omx_set_solver", "omx_set_cg_tables", "omx_build_cg_tables", "omx_update", "omx_solve",
            "omx_cross_sections", "omx_scattering_size", "omx_get_stats", "omx_read_matrix", "omx_read_vector",
-           "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_solve_sweep"]
+           "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_solve_sweep", "omx_mem_info", "omx_trim"]
 
 
 class OmxError(RuntimeError):
@@ -43,7 +43,8 @@ class Stats(ctypes.Structure):
                 ("device_bytes", ctypes.c_size_t), ("t_lu_gemm_ms", ctypes.c_double),
                 ("lu_gemm_flop", ctypes.c_double), ("lu_gemm_launches", ctypes.c_int),
                 ("t_lu_panel_ms", ctypes.c_double), ("t_factor_total_ms", ctypes.c_double),
-                ("pivot_fallbacks", ctypes.c_int), ("sweep_batch", ctypes.c_int)]
+                ("pivot_fallbacks", ctypes.c_int), ("sweep_batch", ctypes.c_int), ("matvec_calls", ctypes.c_int),
+                ("t_matvec_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -81,6 +82,9 @@ def load_library(path=LIB_PATH):
     L.omx_set_profile.argtypes = [P, I]
     L.omx_set_pivoting.argtypes = [P, I]
     L.omx_solve_sweep.argtypes = [P, I, D, D, D, I, D, D, D, D, D, ctypes.POINTER(ctypes.c_int)]
+    SZ = ctypes.POINTER(ctypes.c_size_t)
+    L.omx_mem_info.argtypes = [P, SZ, SZ, SZ]
+    L.omx_trim.argtypes = [P]
     _lib = L
     return L
 
@@ -125,6 +129,16 @@ class Context:
         s = Stats()
         _check(self.L.omx_get_stats(self.h, ctypes.byref(s)))
         return s.as_dict()
+
+    def mem_info(self):
+        """(free, total, held) device bytes."""
+        f, t, h = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+        _check(self.L.omx_mem_info(self.h, ctypes.byref(f), ctypes.byref(t), ctypes.byref(h)))
+        return f.value, t.value, h.value
+
+    def trim(self):
+        """Release the matrix-sized device buffers."""
+        _check(self.L.omx_trim(self.h))
 
     def set_pivoting(self, mode):
         """"block" (default) or "partial" (OMX_PIVOT_*)."""

@@ -1,0 +1,116 @@
+"""Request loop: admission by device-memory footprint (CPU, with a stand-in
+solver) and, on the GPU, that every request's result is the same alone and
+in a concurrent mix."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from optimet_amd.problem import drude_cluster
+from optimet_amd.serving import Request, Server, footprint, summarize
+
+
+def _req(rid, arrival, P=3, nmax=3, nl=2, seed=1):
+    c = drude_cluster(P, nmax, 700e-9, 40, 80, 500, 10, seed=seed + rid)
+    return Request(rid, arrival, c, np.linspace(500e-9, 800e-9, nl))
+
+
+class _Fake:
+    """Stand-in solver: sleeps, records how much footprint is held at once."""
+
+    def __init__(self, server, dt=0.05):
+        self.server, self.dt = server, dt
+        self.lock = threading.Lock()
+        self.running = 0
+        self.peak_running = 0
+        self.peak_held = 0
+        self.order = []
+
+    def __call__(self, slot, req, batch):
+        with self.lock:
+            self.running += 1
+            self.order.append(req.id)
+            self.peak_running = max(self.peak_running, self.running)
+            self.peak_held = max(self.peak_held, self.server._held)
+        time.sleep(self.dt)
+        with self.lock:
+            self.running -= 1
+        return {"slot": slot, "batch": batch}
+
+
+def _server(slots, budget):
+    s = Server(slots=slots, budget=budget, solve_fn=lambda *a: None)
+    fake = _Fake(s)
+    s._solve_fn = fake
+    return s, fake
+
+
+def test_admission_respects_budget_and_order():
+    reqs = [_req(i, 0.0) for i in range(6)]
+    one = footprint(reqs[0].cluster, 2, "lu32ir")
+    s, fake = _server(slots=4, budget=int(2.5 * one))  # room for two requests at a time
+    res = s.run(reqs, realtime=False)
+    assert all(r.error is None for r in res)
+    assert fake.peak_held <= s.budget
+    assert fake.peak_running == 2
+    assert fake.order == sorted(fake.order)  # first come, first served
+
+
+def test_slots_bound_concurrency_and_arrivals_are_honoured():
+    reqs = [_req(i, 0.02 * i) for i in range(5)]
+    s, fake = _server(slots=2, budget=1 << 50)
+    res = s.run(reqs, realtime=True)
+    assert fake.peak_running == 2
+    for r in res:
+        assert r.admitted >= r.arrival - 1e-3 and r.finished > r.admitted
+    summ = summarize(res, {r.id: 2 for r in reqs})
+    assert summ["completed"] == 5 and summ["p99_s"] >= summ["p50_s"] > 0
+
+
+def test_oversize_request_is_rejected_not_queued_forever():
+    big = _req(0, 0.0, P=40, nmax=8)
+    small = _req(1, 0.0)
+    s, fake = _server(slots=2, budget=footprint(small.cluster, 2, "lu32ir") + 1)
+    res = s.run([big, small], realtime=False)
+    assert res[0].error is not None and "exceeds" in res[0].error
+    assert res[1].error is None
+
+
+def test_batch_shrinks_to_fit():
+    req = _req(0, 0.0, nl=16)
+    per = footprint(req.cluster, 1, "lu32ir")
+    s = Server(slots=1, budget=5 * per, solve_fn=lambda *a: None)
+    b, nbytes = s.plan(req)
+    assert b == 4 and nbytes <= s.budget
+
+
+@pytest.mark.gpu
+def test_results_same_alone_and_in_the_mix():
+    """Each request solved alone (one slot, one at a time) and in a mix of
+    concurrent requests (three slots, all arriving together) gives the same
+    coefficients and cross sections to 1e-10 (the same kernels and
+    iterations run; only their interleaving on the device differs)."""
+    reqs = [_req(0, 0.0, P=4, nmax=4, nl=3), _req(1, 0.0, P=6, nmax=3, nl=2), _req(2, 0.0, P=3, nmax=5, nl=4),
+            _req(3, 0.0, P=5, nmax=4, nl=1)]
+    alone = {}
+    srv = Server(slots=1, budget=8 << 30, kind="lu32ir")
+    try:
+        for r in reqs:
+            (res,) = srv.run([r], realtime=False)
+            assert res.error is None, res.error
+            alone[r.id] = res.out
+    finally:
+        srv.close()
+    srv = Server(slots=3, budget=8 << 30, kind="lu32ir")
+    try:
+        mixed = srv.run(reqs, realtime=False)
+    finally:
+        srv.close()
+    for res in mixed:
+        assert res.error is None, res.error
+        a = alone[res.id]
+        for key in ("x_sca", "x_sca_sh"):
+            d = np.linalg.norm(res.out[key] - a[key]) / np.linalg.norm(a[key])
+            assert d < 1e-10, (res.id, key, d)
+        assert np.allclose(res.out["xs"], a["xs"], rtol=1e-10, atol=0)
