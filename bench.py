@@ -215,7 +215,8 @@ def config5_main(args):
     """Config 5: the seeded Poisson request stream (P ~ U{2..128}, nmax ~
     U{4..10}, 1-64 wavelengths, Drude or constant permittivity, omega and
     2 omega) served in real time by the request loop; a bounded sample: the
-    first `--requests` requests, admission closed after `--deadline` s."""
+    first `--requests` requests (optionally, admission closed after
+    `--deadline` s)."""
     from optimet_amd.problem import serving_requests
     from optimet_amd.serving import Request, Server, summarize
 
@@ -229,7 +230,10 @@ def config5_main(args):
             res = srv.run(reqs, realtime=True, deadline=args.deadline)
         finally:
             srv.close()
-        out[kind] = (summarize(res, nl), res)
+        summ = summarize(res, nl)
+        summ["error_kinds"] = sorted({r.error.split("(")[0] for r in res if r.error})
+        summ["latency_s"] = {r.id: r.latency for r in res if r.error is None}
+        out[kind] = (summ, res)
     m = out["lu32ir"][0]
     line = {
         "metric": "cluster solves/sec + p99 latency, mixed-precision vs FP64 LU at rel err <=1e-9",
@@ -274,9 +278,9 @@ def main():
     ap.add_argument("--workload", choices=("config2", "config3", "config4", "config5"), default="config2",
                     help="config2: one 64-sphere cluster (headline); config3: 256-wavelength batched sweep; "
                          "config4: 512 spheres, N = 49152, GMRES vs mixed LU; config5: request stream")
-    ap.add_argument("--requests", type=int, default=16, help="config5: requests of the stream served")
-    ap.add_argument("--slots", type=int, default=3, help="config5: concurrent requests")
-    ap.add_argument("--deadline", type=float, default=40.0, help="config5: admission closes after this many s")
+    ap.add_argument("--requests", type=int, default=8, help="config5: requests of the stream served")
+    ap.add_argument("--slots", type=int, default=6, help="config5: concurrent requests")
+    ap.add_argument("--deadline", type=float, default=None, help="config5: admission closes after this many s")
     ap.add_argument("--batch", type=int, default=0, help="systems per batch of the config3 sweep (0: automatic)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer (calibration of rocprofv3 FETCH_SIZE/WRITE_SIZE)")

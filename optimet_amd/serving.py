@@ -12,7 +12,8 @@ on the GPU.
 Admission is first-come-first-served. A request starts as soon as a slot is
 free and its HBM footprint fits in what the admitted requests leave of the
 budget: requests that arrive while others run are admitted as slots and
-memory free up.
+memory free up. While the oldest request waits for memory, a later one whose
+footprint is at most an eighth of the budget may start if it fits.
 
 Inside a request the wavelengths are solved as batched sweeps
 (`Solver.sweep`, i.e. `omx_solve_sweep`). The T-matrices, the assembled S and V
@@ -152,10 +153,28 @@ class Server:
                     res.admitted = res.finished = now
                     pending.popleft()
                     continue
-                if not self._free_slots or self._held + nbytes > self.budget:
+                if not self._free_slots:
                     self._lock.wait(timeout=0.05)
                     continue
-                pending.popleft()
+                if self._held + nbytes > self.budget:
+                    # the head waits for memory: a small request that has
+                    # arrived and fits now may start beside the running ones
+                    # (at most budget / 8, so the head is not starved)
+                    small = None
+                    for cand in list(pending)[1:]:
+                        if realtime and cand.arrival > now:
+                            break
+                        b2, n2 = self.plan(cand)
+                        if n2 <= self.budget // 8 and self._held + n2 <= self.budget:
+                            small = (cand, b2, n2)
+                            break
+                    if small is None:
+                        self._lock.wait(timeout=0.05)
+                        continue
+                    req, batch, nbytes = small
+                    pending.remove(req)
+                else:
+                    pending.popleft()
                 slot = self._free_slots.pop()
                 self._held += nbytes
                 res = results[req.id]

@@ -114,3 +114,15 @@ def test_results_same_alone_and_in_the_mix():
             d = np.linalg.norm(res.out[key] - a[key]) / np.linalg.norm(a[key])
             assert d < 1e-10, (res.id, key, d)
         assert np.allclose(res.out["xs"], a["xs"], rtol=1e-10, atol=0)
+
+
+def test_small_request_backfills_while_head_waits_for_memory():
+    big = [_req(i, 0.0, P=6, nmax=4) for i in range(2)]
+    small = _req(2, 0.0, P=2, nmax=2, nl=1)
+    fb = footprint(big[0].cluster, 2, "lu32ir")
+    s, fake = _server(slots=3, budget=int(1.5 * fb))  # one big request at a time
+    fake.dt = 0.1
+    res = s.run(big + [small], realtime=False)
+    assert all(r.error is None for r in res)
+    # the small request started before the second big one
+    assert res[2].admitted < res[1].admitted
