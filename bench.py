@@ -68,6 +68,10 @@ def pct(v, q):
 
 
 def run_path(cluster, kind, steps, warmup, profile):
+    """Times `steps` unprofiled steps (the value and latencies), then, with
+    `profile`, runs one more step with per-launch HIP events on (which
+    serialises the omega and 2 omega factorisations) for the stage and GEMM
+    figures."""
     from optimet_amd import solver
 
     ctx = solver.Context()
@@ -76,7 +80,6 @@ def run_path(cluster, kind, steps, warmup, profile):
         s.update()
         s.solve()
         s.cross_sections()
-    ctx.set_profile(profile)
     lat = []
     out = cs = None
     t_start = time.perf_counter()
@@ -88,7 +91,13 @@ def run_path(cluster, kind, steps, warmup, profile):
         lat.append(time.perf_counter() - t0)
     total = time.perf_counter() - t_start
     st = s.stats()
-    ctx.set_profile(False)
+    if profile:
+        ctx.set_profile(True)
+        s.update()
+        s.solve()
+        s.cross_sections()
+        st = s.stats()
+        ctx.set_profile(False)
     ctx.close()
     return {"lat": lat, "total": total, "out": out, "cs": cs, "stats": st}
 

@@ -96,6 +96,14 @@ struct omx_context {
   DBuf d_piv, d_perm, d_cand, d_cnt, d_map, d_status, d_flags, d_wlu;
   hipStream_t aux = nullptr;
   hipEvent_t lu_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // the second-harmonic system's own streams, events, status and maps, so
+  // its factorisation can run beside the fundamental solve
+  hipStream_t st2 = nullptr, aux2 = nullptr;
+  hipEvent_t lu_ev2[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t xev[4] = {nullptr, nullptr, nullptr, nullptr};
+  DBuf d_status2, d_map2;
+  std::vector<hipEvent_t> pev2;
+  omx::LuProf prof2{};
   DBuf d_basis, d_w1, d_w2, d_r, d_part, d_h, d_nrm, d_gpart;
   omx::LuWork lu{}, lu_sh{};
   bool profile = false;
@@ -154,9 +162,14 @@ void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags
     c->d_map.ensure(sizeof(int) * (3 * ((size_t)Np + 2 * omx::kNB)), &c->total);
     c->d_status.ensure(64, &c->total);
     c->d_wlu.ensure(sizeof(cplx) * 3 * omx::kNB * omx::kNB, &c->total);
+  } else {
+    c->d_map2.ensure(sizeof(int) * (3 * ((size_t)Np + 2 * omx::kNB)), &c->total);
+    c->d_status2.ensure(64, &c->total);
   }
+  DBuf &map = second ? c->d_map2 : c->d_map;
+  DBuf &status = second ? c->d_status2 : c->d_status;
   HIPCHK(hipMemsetAsync(flags.p, 0, sizeof(int) * (Np / omx::kNB + 1), c->st));
-  HIPCHK(hipMemsetAsync(c->d_status.p, 0, 64, c->st));
+  HIPCHK(hipMemsetAsync(status.p, 0, 64, c->st));
   w = omx::LuWork{};
   w.piv = piv.as<int>();
   w.perm = perm.as<int>();
@@ -165,18 +178,19 @@ void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags
   w.cand[1] = c->d_cand.as<int>() + gmax * omx::kNB;
   w.cnt[0] = c->d_cnt.as<int>();
   w.cnt[1] = c->d_cnt.as<int>() + gmax;
-  w.mdst = c->d_map.as<int>();
-  w.msrc = c->d_map.as<int>() + (Np + 2 * omx::kNB);
-  w.mcnt = c->d_map.as<int>() + 2 * (Np + 2 * omx::kNB);
-  w.status = c->d_status.as<int>();
+  w.mdst = map.as<int>();
+  w.msrc = map.as<int>() + (Np + 2 * omx::kNB);
+  w.mcnt = map.as<int>() + 2 * (Np + 2 * omx::kNB);
+  w.status = status.as<int>();
   w.wlu = c->d_wlu.p;
   w.dinv = dinv.p;
   w.pivot = c->pivot;
-  w.aux = c->aux;
-  w.ev_next = c->lu_ev[0];
-  w.ev_piv = c->lu_ev[1];
-  w.ev_bulk = c->lu_ev[2];
-  w.ev_pan = c->lu_ev[3];
+  hipEvent_t *ev = second ? c->lu_ev2 : c->lu_ev;
+  w.aux = second ? c->aux2 : c->aux;
+  w.ev_next = ev[0];
+  w.ev_piv = ev[1];
+  w.ev_bulk = ev[2];
+  w.ev_pan = ev[3];
   w.gmax = gmax;
   w.epoch = 0;
 }
@@ -253,19 +267,20 @@ struct Gmres {
   const cplx *A;  // FP64 operator
   std::function<void(const cplx *, cplx *)> prec;
   int N;
+  hipStream_t s;  // the stream every step runs on
   double nrm(const cplx *x) {
-    omx::nrm2sq(x, N, c->d_nrm.as<double>() + 1, c->d_nrm.as<double>(), c->st);
+    omx::nrm2sq(x, N, c->d_nrm.as<double>() + 1, c->d_nrm.as<double>(), s);
     double v;
-    HIPCHK(hipMemcpyAsync(&v, c->d_nrm.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpyAsync(&v, c->d_nrm.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     return std::sqrt(v);
   }
   void matvec(const cplx *x, cplx *y) {
-    if(c->profile) HIPCHK(hipEventRecord(c->ev[6], c->st));
-    omx::gemv64(A, (size_t)c->Np, N, x, y, c->d_gpart.as<cplx>(), c->st);
+    if(c->profile) HIPCHK(hipEventRecord(c->ev[6], s));
+    omx::gemv64(A, (size_t)c->Np, N, x, y, c->d_gpart.as<cplx>(), s);
     ++c->mv_calls;
     if(c->profile) {
-      HIPCHK(hipEventRecord(c->ev[7], c->st));
+      HIPCHK(hipEventRecord(c->ev[7], s));
       HIPCHK(hipEventSynchronize(c->ev[7]));
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
@@ -281,7 +296,7 @@ struct Gmres {
     const size_t ld = (size_t)c->Np;
     double bn = nrm(b);
     if(bn == 0.0) {
-      omx::vaxpby(x, omx::mk(0.0), x, omx::mk(0.0), N, c->st);
+      omx::vaxpby(x, omx::mk(0.0), x, omx::mk(0.0), N, s);
       *res = 0.0;
       return 0;
     }
@@ -291,7 +306,7 @@ struct Gmres {
     std::vector<std::complex<double>> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m);
     while(true) {
       matvec(x, r);
-      omx::vsub(r, b, r, N, c->st);
+      omx::vsub(r, b, r, N, s);
       double beta = nrm(r);
       *res = beta / bn;
       if(*res <= tol || it >= maxit || tiny_step) break;
@@ -302,7 +317,7 @@ struct Gmres {
         stall = 0;
       }
       prev = beta;
-      omx::vaxpby(V, omx::mk(1.0 / beta), r, omx::mk(0.0), N, c->st);
+      omx::vaxpby(V, omx::mk(1.0 / beta), r, omx::mk(0.0), N, s);
       std::fill(g.begin(), g.end(), 0.0);
       g[0] = beta;
       int j = 0;
@@ -316,16 +331,16 @@ struct Gmres {
         }
         std::vector<cplx> h1(j + 1), h2(j + 1);
         for(int pass = 0; pass < 2; ++pass) {
-          omx::dotc_multi(V, ld, j + 1, w, N, part, hd, c->st);
-          omx::axpy_multi(V, ld, j + 1, hd, w, N, c->st);
+          omx::dotc_multi(V, ld, j + 1, w, N, part, hd, s);
+          omx::axpy_multi(V, ld, j + 1, hd, w, N, s);
           HIPCHK(hipMemcpyAsync(pass ? h2.data() : h1.data(), hd, sizeof(cplx) * (j + 1), hipMemcpyDeviceToHost,
-                                c->st));
+                                s));
         }
         double hn = nrm(w);
         for(int i = 0; i <= j; ++i)
           H[(size_t)j * (m + 1) + i] = std::complex<double>(h1[i].x + h2[i].x, h1[i].y + h2[i].y);
         H[(size_t)j * (m + 1) + j + 1] = hn;
-        if(hn > 0.0) omx::vaxpby(V + (size_t)(j + 1) * ld, omx::mk(1.0 / hn), w, omx::mk(0.0), N, c->st);
+        if(hn > 0.0) omx::vaxpby(V + (size_t)(j + 1) * ld, omx::mk(1.0 / hn), w, omx::mk(0.0), N, s);
         auto *hc = &H[(size_t)j * (m + 1)];
         for(int i = 0; i < j; ++i) {
           std::complex<double> t = cs[i] * hc[i] + sn[i] * hc[i + 1];
@@ -360,14 +375,14 @@ struct Gmres {
       }
       std::vector<cplx> yh(j);
       for(int i = 0; i < j; ++i) yh[i] = omx::mk(y[i].real(), y[i].imag());
-      HIPCHK(hipMemcpyAsync(hd, yh.data(), sizeof(cplx) * j, hipMemcpyHostToDevice, c->st));
-      omx::gemv_multi(V, ld, j, hd, z, N, c->st);
+      HIPCHK(hipMemcpyAsync(hd, yh.data(), sizeof(cplx) * j, hipMemcpyHostToDevice, s));
+      omx::gemv_multi(V, ld, j, hd, z, N, s);
       const cplx *dx = z;
       if(prec) {
         prec(z, w);
         dx = w;
       }
-      omx::vaxpby(x, omx::mk(1.0), dx, omx::mk(1.0), N, c->st);
+      omx::vaxpby(x, omx::mk(1.0), dx, omx::mk(1.0), N, s);
       // a correction below the FP64 resolution of x ends the refinement
       if(nrm(dx) <= 1e-16 * nrm(x)) tiny_step = true;
     }
@@ -375,57 +390,96 @@ struct Gmres {
   }
 };
 
-void check_status(omx_context *c) {
+hipStream_t sys_stream(const omx_context *c, bool shf) { return shf ? c->st2 : c->st; }
+
+void check_status(omx_context *c, bool shf = false) {
   int s[2] = {0, 0};
-  HIPCHK(hipMemcpyAsync(s, c->d_status.p, sizeof(s), hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
+  hipStream_t st = sys_stream(c, shf);
+  HIPCHK(hipMemcpyAsync(s, (shf ? c->d_status2 : c->d_status).p, sizeof(s), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   if(s[1]) throw OmxError(OMX_ERR_DEVICE, "device wait timed out (code " + std::to_string(s[1]) + ")");
   if(s[0]) throw OmxError(OMX_ERR_SINGULAR, "zero pivot at column " + std::to_string(s[0] - 1));
 }
 
-void prof_begin(omx_context *c) {
-  if(!c->profile) return;
+omx::LuProf *prof_begin(omx_context *c, bool shf) {
+  if(!c->profile) return nullptr;
+  std::vector<hipEvent_t> &pev = shf ? c->pev2 : c->pev;
+  omx::LuProf &prof = shf ? c->prof2 : c->prof;
   int cap = c->Np / omx::kNB + 1;
-  if((int)c->pev.size() < 4 * cap + 2) {
-    for(auto e : c->pev) (void)hipEventDestroy(e);
-    c->pev.assign(4 * cap + 2, nullptr);
-    for(auto &e : c->pev) HIPCHK(hipEventCreate(&e));
+  if((int)pev.size() < 4 * cap + 2) {
+    for(auto e : pev) (void)hipEventDestroy(e);
+    pev.assign(4 * cap + 2, nullptr);
+    for(auto &e : pev) HIPCHK(hipEventCreate(&e));
   }
-  c->prof.ev = c->pev.data() + 2;
-  c->prof.cap = cap;
-  HIPCHK(hipEventRecord(c->pev[0], c->st));
+  prof.ev = pev.data() + 2;
+  prof.cap = cap;
+  HIPCHK(hipEventRecord(pev[0], sys_stream(c, shf)));
+  return &prof;
 }
-void prof_end(omx_context *c) {
+void prof_end(omx_context *c, bool shf) {
   if(!c->profile) return;
-  HIPCHK(hipEventRecord(c->pev[1], c->st));
-  HIPCHK(hipEventSynchronize(c->pev[1]));
+  std::vector<hipEvent_t> &pev = shf ? c->pev2 : c->pev;
+  omx::LuProf &prof = shf ? c->prof2 : c->prof;
+  HIPCHK(hipEventRecord(pev[1], sys_stream(c, shf)));
+  HIPCHK(hipEventSynchronize(pev[1]));
   float ms = 0;
-  for(int i = 0; i < c->prof.used; ++i) {
-    hipEvent_t *e = c->prof.ev + 4 * i;
+  for(int i = 0; i < prof.used; ++i) {
+    hipEvent_t *e = prof.ev + 4 * i;
     HIPCHK(hipEventElapsedTime(&ms, e[0], e[1]));
     c->prof_panel_ms += ms;
     HIPCHK(hipEventElapsedTime(&ms, e[2], e[3]));
     c->prof_gemm_ms += ms;
   }
-  HIPCHK(hipEventElapsedTime(&ms, c->pev[0], c->pev[1]));
+  HIPCHK(hipEventElapsedTime(&ms, pev[0], pev[1]));
   c->prof_total_ms += ms;
-  c->prof_flop += c->prof.gemm_flop;
-  c->prof_launches += c->prof.gemm_launches;
+  c->prof_flop += prof.gemm_flop;
+  c->prof_launches += prof.gemm_launches;
 }
 
-// Solves (S or V) x = rhs with the context's solver kind.
+// Enqueues the factorisation of S (V) with pivot mode pv on the system's
+// stream; the status is checked by the caller.
+void factor_enqueue(omx_context *c, bool shf, int pv) {
+  omx::LuWork &lu = shf ? c->lu_sh : c->lu;
+  cplx *A64 = (shf ? c->d_V64 : c->d_S64).as<cplx>();
+  const int Np = c->Np;
+  const size_t lda = (size_t)Np, n2 = lda * Np;
+  hipStream_t st = sys_stream(c, shf);
+  HIPCHK(hipMemsetAsync((shf ? c->d_status2 : c->d_status).p, 0, 16, st));
+  lu.pivot = pv;
+  omx::LuProf *prof = prof_begin(c, shf);
+  int rc;
+  if(c->kind == OMX_SOLVER_LU64) {
+    cplx *F64 = (shf ? c->d_F64sh : c->d_F64).as<cplx>();
+    HIPCHK(hipMemcpyAsync(F64, A64, sizeof(cplx) * n2, hipMemcpyDeviceToDevice, st));
+    rc = omx::lu_factor<double2>(reinterpret_cast<double2 *>(F64), Np, lda, lu, st, prof);
+  } else {
+    float2 *A32 = (shf ? c->d_V32 : c->d_S32).as<float2>();
+    bool &dirty = shf ? c->dirty32_sh : c->dirty32;
+    if(dirty) omx::demote(A32, A64, n2, st);  // an earlier factorisation overwrote it
+    dirty = true;
+    rc = omx::lu_factor<float2>(A32, Np, lda, lu, st, prof);
+  }
+  if(rc) throw OmxError(OMX_ERR_DEVICE, "lu_factor launch failed (" + std::to_string(rc) + ")");
+  prof_end(c, shf);
+  (shf ? c->factored_sh : c->factored) = true;
+}
+
+// Solves (S or V) x = rhs with the context's solver kind, on the system's
+// stream (a factorisation already enqueued by factor_enqueue is used).
 void solve_system(omx_context *c, bool shf, const cplx *rhs, cplx *x, int *iters, double *res) {
   omx::LuWork &lu = shf ? c->lu_sh : c->lu;
   cplx *A64 = (shf ? c->d_V64 : c->d_S64).as<cplx>();
   bool &fact = shf ? c->factored_sh : c->factored;
   const int Np = c->Np;
-  const size_t lda = (size_t)Np, n2 = lda * Np;
-  Gmres gm{c, A64, nullptr, Np};
+  const size_t lda = (size_t)Np;
+  hipStream_t st = sys_stream(c, shf);
+  DBuf &status = shf ? c->d_status2 : c->d_status;
+  Gmres gm{c, A64, nullptr, Np, st};
   if(c->kind == OMX_SOLVER_GMRES64) {
-    HIPCHK(hipMemsetAsync(c->d_status.p, 0, 16, c->st));
-    HIPCHK(hipMemsetAsync(x, 0, sizeof(cplx) * Np, c->st));
+    HIPCHK(hipMemsetAsync(status.p, 0, 16, st));
+    HIPCHK(hipMemsetAsync(x, 0, sizeof(cplx) * Np, st));
     *iters = gm.solve(rhs, x, c->tol, c->maxit, c->restart, res);
-    check_status(c);
+    check_status(c, shf);
     if(!(*res <= 1e3 * c->tol))
       throw OmxError(OMX_ERR_NOCONV, "GMRES stopped at relative residual " + std::to_string(*res));
     return;
@@ -435,49 +489,31 @@ void solve_system(omx_context *c, bool shf, const cplx *rhs, cplx *x, int *iters
   float2 *A32 = (shf ? c->d_V32 : c->d_S32).as<float2>();
   // Block pivoting first (when selected); a zero pivot or a refinement that
   // does not converge falls back to tournament (partial) pivoting.
-  bool &dirty = shf ? c->dirty32_sh : c->dirty32;
   const int first = fact ? lu.pivot : c->pivot;
   for(int pv = first; pv <= OMX_PIVOT_PARTIAL; ++pv) {
     const bool last = pv == OMX_PIVOT_PARTIAL;
     try {
-      HIPCHK(hipMemsetAsync(c->d_status.p, 0, 16, c->st));
-      if(!fact || lu.pivot != pv) {
-        lu.pivot = pv;
-        prof_begin(c);
-        int rc;
-        if(f64) {
-          HIPCHK(hipMemcpyAsync(F64, A64, sizeof(cplx) * n2, hipMemcpyDeviceToDevice, c->st));
-          rc = omx::lu_factor<double2>(reinterpret_cast<double2 *>(F64), Np, lda, lu, c->st,
-                                       c->profile ? &c->prof : nullptr);
-        } else {
-          if(dirty) omx::demote(A32, A64, n2, c->st);  // an earlier factorisation overwrote it
-          dirty = true;
-          rc = omx::lu_factor<float2>(A32, Np, lda, lu, c->st, c->profile ? &c->prof : nullptr);
-        }
-        if(rc) throw OmxError(OMX_ERR_DEVICE, "lu_factor launch failed (" + std::to_string(rc) + ")");
-        prof_end(c);
-        fact = true;
-        check_status(c);
-      }
+      if(!fact || lu.pivot != pv) factor_enqueue(c, shf, pv);
+      check_status(c, shf);  // zero pivots of this factorisation
       if(f64) {
-        gm.prec = [c, F64, &lu, Np, lda](const cplx *in, cplx *out) {
-          omx::lu_solve<double2>(reinterpret_cast<const double2 *>(F64), Np, lda, lu, in, out, c->st);
+        gm.prec = [F64, &lu, Np, lda, st](const cplx *in, cplx *out) {
+          omx::lu_solve<double2>(reinterpret_cast<const double2 *>(F64), Np, lda, lu, in, out, st);
         };
-        omx::lu_solve<double2>(reinterpret_cast<const double2 *>(F64), Np, lda, lu, rhs, x, c->st);
+        omx::lu_solve<double2>(reinterpret_cast<const double2 *>(F64), Np, lda, lu, rhs, x, st);
         if(pv == OMX_PIVOT_PARTIAL) {  // plain FP64 LU with partial pivoting
-          check_status(c);
+          check_status(c, shf);
           *iters = 0;
           *res = -1.0;
           return;
         }
       } else {
-        gm.prec = [c, A32, &lu, Np, lda](const cplx *in, cplx *out) {
-          omx::lu_solve<float2>(A32, Np, lda, lu, in, out, c->st);
+        gm.prec = [A32, &lu, Np, lda, st](const cplx *in, cplx *out) {
+          omx::lu_solve<float2>(A32, Np, lda, lu, in, out, st);
         };
-        omx::lu_solve<float2>(A32, Np, lda, lu, rhs, x, c->st);  // x0 = M^{-1} b
+        omx::lu_solve<float2>(A32, Np, lda, lu, rhs, x, st);  // x0 = M^{-1} b
       }
       *iters = gm.solve(rhs, x, c->tol, c->maxit, c->restart, res);
-      check_status(c);
+      check_status(c, shf);
       // the target may sit below the FP64 residual floor of an ill-conditioned
       // system; a stagnated refinement within 1000x of it is accepted
       if(!(*res <= 1e3 * c->tol))
@@ -932,7 +968,11 @@ int omx_create(omx_context **out, int device) {
       HIPCHK(hipSetDevice(device));
       HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
       HIPCHK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+      HIPCHK(hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
+      HIPCHK(hipStreamCreateWithFlags(&c->aux2, hipStreamNonBlocking));
       for(auto &e : c->lu_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      for(auto &e : c->lu_ev2) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      for(auto &e : c->xev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       for(auto &e : c->ev) HIPCHK(hipEventCreate(&e));
       c->d_coef.ensure(sizeof(double) * 6 * omx::kCoefStride, &c->total);
       std::vector<double> h(6 * omx::kCoefStride, 0.0);
@@ -951,6 +991,9 @@ void omx_destroy(omx_context *c) {
   if(!c) return;
   (void)hipSetDevice(c->device);
   if(c->st) (void)hipStreamSynchronize(c->st);
+  if(c->st2) (void)hipStreamSynchronize(c->st2);
+  c->d_status2.release(&c->total);
+  c->d_map2.release(&c->total);
   DBuf *bufs[] = {&c->d_xyz, &c->d_radius, &c->d_eps, &c->d_mu, &c->d_eps_sh, &c->d_kppp, &c->d_kpar, &c->d_gam,
                   &c->d_sph, &c->d_ov, &c->d_inc, &c->d_incloc, &c->d_Q, &c->d_w, &c->d_coef, &c->d_S64,
                   &c->d_S32, &c->d_V64, &c->d_V32, &c->d_x, &c->d_xsh, &c->d_xint, &c->d_xintsh, &c->d_K,
@@ -971,6 +1014,14 @@ void omx_destroy(omx_context *c) {
     if(e) (void)hipEventDestroy(e);
   for(auto e : c->lu_ev)
     if(e) (void)hipEventDestroy(e);
+  for(auto e : c->lu_ev2)
+    if(e) (void)hipEventDestroy(e);
+  for(auto e : c->xev)
+    if(e) (void)hipEventDestroy(e);
+  for(auto e : c->pev2)
+    if(e) (void)hipEventDestroy(e);
+  if(c->aux2) (void)hipStreamDestroy(c->aux2);
+  if(c->st2) (void)hipStreamDestroy(c->st2);
   if(c->aux) (void)hipStreamDestroy(c->aux);
   if(c->st) (void)hipStreamDestroy(c->st);
   delete c;
@@ -1155,6 +1206,16 @@ int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, do
     const int N = c->N, P = c->P, nmax = c->nmax;
     double tf = 0, ts = 0, tsh = 0;
     HIPCHK(hipEventRecord(c->ev[0], c->st));
+    // V does not depend on the fundamental solution, only its right-hand
+    // side does: its factorisation runs on the second stream pair beside the
+    // fundamental solve (not while profiling, which times each factorisation
+    // alone)
+    const bool lu_kind = c->kind != OMX_SOLVER_GMRES64;
+    if(c->sh && lu_kind && !c->factored_sh && !c->profile) {
+      HIPCHK(hipEventRecord(c->xev[0], c->st));
+      HIPCHK(hipStreamWaitEvent(c->st2, c->xev[0], 0));
+      factor_enqueue(c, true, c->pivot);
+    }
     // X_sca = S^{-1} Q (PreconditionedMatrixSolver.h:58)
     solve_system(c, false, c->d_Q.as<cplx>(), c->d_x.as<cplx>(), &c->stats.iterations, &c->stats.residual);
     HIPCHK(hipEventRecord(c->ev[1], c->st));
@@ -1178,10 +1239,15 @@ int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, do
                      c->d_xint.as<cplx>(), c->d_K.as<cplx>(), c->d_K1.as<cplx>()};
       omx::run_shsrc(sa, c->st);
       HIPCHK(hipEventRecord(c->ev[4], c->st));
+      // the second-harmonic solve runs on the second stream, after the source
+      HIPCHK(hipEventRecord(c->xev[1], c->st));
+      HIPCHK(hipStreamWaitEvent(c->st2, c->xev[1], 0));
       solve_system(c, true, c->d_K.as<cplx>(), c->d_xsh.as<cplx>(), &c->stats.iterations_sh, &c->stats.residual_sh);
       // X_int_SH = X_sca_SH (.) IauxSH1 - K1ana (Solver.cpp:95-116)
       omx::vmul_ov(c->d_xintsh.as<cplx>(), c->d_xsh.as<cplx>(), c->d_ov.as<cplx>(), omx::OV_IAUXSH1, P, nmax,
-                   c->d_K1.as<cplx>(), N, c->st);
+                   c->d_K1.as<cplx>(), N, c->st2);
+      HIPCHK(hipEventRecord(c->xev[2], c->st2));
+      HIPCHK(hipStreamWaitEvent(c->st, c->xev[2], 0));
       HIPCHK(hipEventRecord(c->ev[5], c->st));
       HIPCHK(hipEventSynchronize(c->ev[5]));
       tsh = elapsed(c, 3, 4);
