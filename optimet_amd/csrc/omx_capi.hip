@@ -8,6 +8,7 @@
 #include <cmath>
 #include <complex>
 #include <functional>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -82,6 +83,9 @@ struct omx_context {
   // solver
   int kind = OMX_SOLVER_LU64;
   int pivot = OMX_PIVOT_BLOCK;
+  // fraction of S's columns factored before V's factorisation starts
+  // (OMX_SH_OVERLAP overrides; 0 starts both together)
+  double sh_overlap = 0.5;
   int fallbacks = 0;
   double tol = 1e-16;
   int maxit = 300, restart = 40;
@@ -156,12 +160,14 @@ void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags
   dinv.ensure(sizeof(cplx) * 2 * omx::kNB * (size_t)Np, &c->total);
   perm.ensure(sizeof(int) * Np, &c->total);
   flags.ensure(sizeof(int) * (Np / omx::kNB + 1), &c->total);
+  // tournament scratch: one half per system, as S and V may be factored
+  // at the same time
+  c->d_cand.ensure(sizeof(int) * 4 * gmax * omx::kNB, &c->total);
+  c->d_cnt.ensure(sizeof(int) * 4 * gmax, &c->total);
+  c->d_wlu.ensure(sizeof(cplx) * 6 * omx::kNB * omx::kNB, &c->total);
   if(!second) {
-    c->d_cand.ensure(sizeof(int) * 2 * gmax * omx::kNB, &c->total);
-    c->d_cnt.ensure(sizeof(int) * 2 * gmax, &c->total);
     c->d_map.ensure(sizeof(int) * (3 * ((size_t)Np + 2 * omx::kNB)), &c->total);
     c->d_status.ensure(64, &c->total);
-    c->d_wlu.ensure(sizeof(cplx) * 3 * omx::kNB * omx::kNB, &c->total);
   } else {
     c->d_map2.ensure(sizeof(int) * (3 * ((size_t)Np + 2 * omx::kNB)), &c->total);
     c->d_status2.ensure(64, &c->total);
@@ -174,15 +180,16 @@ void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags
   w.piv = piv.as<int>();
   w.perm = perm.as<int>();
   w.flags = flags.as<int>();
-  w.cand[0] = c->d_cand.as<int>();
-  w.cand[1] = c->d_cand.as<int>() + gmax * omx::kNB;
-  w.cnt[0] = c->d_cnt.as<int>();
-  w.cnt[1] = c->d_cnt.as<int>() + gmax;
+  const int half = second ? 1 : 0;
+  w.cand[0] = c->d_cand.as<int>() + (2 * half) * gmax * omx::kNB;
+  w.cand[1] = c->d_cand.as<int>() + (2 * half + 1) * gmax * omx::kNB;
+  w.cnt[0] = c->d_cnt.as<int>() + (2 * half) * gmax;
+  w.cnt[1] = c->d_cnt.as<int>() + (2 * half + 1) * gmax;
   w.mdst = map.as<int>();
   w.msrc = map.as<int>() + (Np + 2 * omx::kNB);
   w.mcnt = map.as<int>() + 2 * (Np + 2 * omx::kNB);
   w.status = status.as<int>();
-  w.wlu = c->d_wlu.p;
+  w.wlu = c->d_wlu.as<cplx>() + (size_t)half * 3 * omx::kNB * omx::kNB;
   w.dinv = dinv.p;
   w.pivot = c->pivot;
   hipEvent_t *ev = second ? c->lu_ev2 : c->lu_ev;
@@ -963,6 +970,7 @@ int omx_create(omx_context **out, int device) {
     HIPCHK(hipGetDeviceCount(&n));
     if(device < 0 || device >= n) throw OmxError(OMX_ERR_DEVICE, "no HIP device " + std::to_string(device));
     auto *c = new omx_context();
+    if(const char *e = std::getenv("OMX_SH_OVERLAP")) c->sh_overlap = std::atof(e);
     c->device = device;
     try {
       HIPCHK(hipSetDevice(device));
@@ -1212,7 +1220,23 @@ int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, do
     // alone)
     const bool lu_kind = c->kind != OMX_SOLVER_GMRES64;
     if(c->sh && lu_kind && !c->factored_sh && !c->profile) {
-      HIPCHK(hipEventRecord(c->xev[0], c->st));
+      // V's factorisation starts once S's has passed a fraction of its
+      // columns (by then S's trailing updates no longer fill the device),
+      // and overlaps the rest of it and the fundamental refinement
+      const int mark = (int)(c->sh_overlap * c->Np);
+      if(mark > 0 && !c->factored) {
+        c->lu.ev_mark = c->xev[0];
+        c->lu.mark_col = mark;
+        try {
+          factor_enqueue(c, false, c->pivot);
+        } catch(...) {
+          c->lu.ev_mark = nullptr;
+          throw;
+        }
+        c->lu.ev_mark = nullptr;
+      } else {
+        HIPCHK(hipEventRecord(c->xev[0], c->st));
+      }
       HIPCHK(hipStreamWaitEvent(c->st2, c->xev[0], 0));
       factor_enqueue(c, true, c->pivot);
     }

@@ -972,7 +972,9 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
         (void)hipEventRecord(pe[3], st);
         prof->used = pi + 1;
       }
+      if(w.ev_mark && far >= w.mark_col && far - 2 * kNB < w.mark_col) (void)hipEventRecord(w.ev_mark, st);
     }
+    if(w.ev_mark && Np < w.mark_col) (void)hipEventRecord(w.ev_mark, st);
     if(Np > kNB)
       hipLaunchKernelGGL((laswp_left<T>), dim3((Np - kNB) / 8, Np / kNB - 1, nb), dim3(256), 0, st, A, lda, w.mdst,
                          w.msrc, w.mcnt, bs);
@@ -1050,7 +1052,9 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
       (void)hipEventRecord(pe[3], st);
       prof->used = pi + 1;
     }
+    if(w.ev_mark && p1 + kNB >= w.mark_col && p0 < w.mark_col) (void)hipEventRecord(w.ev_mark, st);
   }
+  if(w.ev_mark && Np < w.mark_col) (void)hipEventRecord(w.ev_mark, st);
   return 0;
 }
 
