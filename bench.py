@@ -28,11 +28,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_MFMA_TFLOPS = 157.3   # MI355X dense FP32 matrix peak
+BF16_MFMA_TFLOPS = 2500.0  # MI355X dense bf16 matrix peak
+# the split-bf16 update issues six bf16 products per real FP32-accurate product
+SPLIT_TFLOPS = BF16_MFMA_TFLOPS / 6
 FP64_MFMA_TFLOPS = 78.6    # MI355X dense FP64 matrix peak
 HBM_GBS = 8000.0
-GEMM_KERNEL = {"lu32ir": "gemm_tile<float2,128,64,128,4,2,16>", "lu64": "gemm_tile<double2,128,64,128,4,2,16>"}
+GEMM_KERNEL = {"lu32ir": "gemm_split<128,128,128,4,2,16,8> (bf16x3 split, complex FP32)", "lu64": "gemm_tile<double2,128,64,128,4,2,16>"}
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                            "traffic_gemm_tile_c64_config2.txt")
+                            "traffic_gemm_split_config2.txt")
 
 
 def traffic_per_launch():
@@ -197,7 +200,7 @@ def config4_main(args):
                      if st["t_matvec_ms"] > 0 else None,
                      "out": out, "cext": cs["cext"], "stats": st}
         if kind == "lu32ir":
-            roof = roofline(st, "lu32ir", FP32_MFMA_TFLOPS)
+            roof = roofline(st, "lu32ir", SPLIT_TFLOPS)
             res[kind]["roofline"] = roof
     g, m = res["gmres64"], res["lu32ir"]
     rel = (float(np.linalg.norm(m["out"]["x_sca"] - g["out"]["x_sca"]) / np.linalg.norm(g["out"]["x_sca"]))
@@ -328,8 +331,9 @@ def main():
 
     ms = mixed["stats"]
     value = args.steps / mixed["total"]
-    roof = roofline(ms, "lu32ir", FP32_MFMA_TFLOPS)
+    roof = roofline(ms, "lu32ir", SPLIT_TFLOPS)
     if roof is not None:
+        roof["peak_basis"] = "bf16 dense 2500 TFLOP/s / 6 bf16 products per FP32-accurate real product"
         roof["traffic"] = traffic_per_launch()
     line = {
         "metric": "cluster solves/sec + p99 latency, mixed-precision vs FP64 LU at rel err <=1e-9",

@@ -58,6 +58,12 @@ extern "C" {
                                  zero pivot or a refinement that does not converge */
 #define OMX_PIVOT_PARTIAL 1   /* partial pivoting over the whole column (tournament) */
 
+/* Products of the complex-FP32 factorisation's bulk updates (omx_set_lu32_gemm). */
+#define OMX_GEMM_BF16X3 0     /* each FP32 operand split in three bf16 terms, six
+                                 bf16 MFMA products per real product, FP32
+                                 accumulation: FP32 accuracy at the bf16 rate */
+#define OMX_GEMM_FP32 1       /* the FP32 matrix cores */
+
 typedef struct omx_context omx_context;
 
 typedef struct {
@@ -121,6 +127,11 @@ int omx_set_solver(omx_context *ctx, int kind, double tol, int maxit, int restar
  * modes reach the same FP64 residual.  OMX_PIVOT_PARTIAL with
  * OMX_SOLVER_LU64 is the plain partial-pivoting FP64 LU solve. */
 int omx_set_pivoting(omx_context *ctx, int mode);
+
+/* Products of OMX_SOLVER_LU32_IR's bulk updates (default OMX_GEMM_BF16X3).
+ * No reference counterpart: the reference factorises in FP64 only; both
+ * modes give an FP32-accurate factorisation, refined to FP64 by GMRES-IR. */
+int omx_set_lu32_gemm(omx_context *ctx, int mode);
 
 /* The nine Clebsch-Gordan / Wigner tables of the second-harmonic source
  * (CGcoeff, Simulation.cpp:613-616), each nf*nf*nf doubles in the order

@@ -86,6 +86,7 @@ struct omx_context {
   // fraction of S's columns factored before V's factorisation starts
   // (OMX_SH_OVERLAP overrides; 0 starts both together)
   double sh_overlap = 0.5;
+  int lu32_gemm = OMX_GEMM_BF16X3;
   int fallbacks = 0;
   double tol = 1e-16;
   int maxit = 300, restart = 40;
@@ -453,6 +454,7 @@ void factor_enqueue(omx_context *c, bool shf, int pv) {
   hipStream_t st = sys_stream(c, shf);
   HIPCHK(hipMemsetAsync((shf ? c->d_status2 : c->d_status).p, 0, 16, st));
   lu.pivot = pv;
+  lu.split = c->lu32_gemm == OMX_GEMM_BF16X3;
   omx::LuProf *prof = prof_begin(c, shf);
   int rc;
   if(c->kind == OMX_SOLVER_LU64) {
@@ -781,6 +783,7 @@ omx::LuWork sweep_lu(omx_context *c, int nb, bool second) {
   lw.gmax = 0;
   lw.epoch = 0;
   lw.pivot = omx::kPivotBlock;
+  lw.split = c->lu32_gemm == OMX_GEMM_BF16X3;
   lw.nb = nb;
   lw.bs.a = (size_t)Np * Np;
   lw.bs.inv = (size_t)2 * omx::kNB * Np;
@@ -1112,6 +1115,15 @@ int omx_set_pivoting(omx_context *c, int mode) {
   return guard([&] {
     if(!c || (mode != OMX_PIVOT_BLOCK && mode != OMX_PIVOT_PARTIAL)) throw OmxError(OMX_ERR_ARG, "invalid pivoting");
     c->pivot = mode;
+    c->factored = c->factored_sh = false;
+    c->solved = false;
+  });
+}
+
+int omx_set_lu32_gemm(omx_context *c, int mode) {
+  return guard([&] {
+    if(!c || (mode != OMX_GEMM_BF16X3 && mode != OMX_GEMM_FP32)) throw OmxError(OMX_ERR_ARG, "invalid gemm mode");
+    c->lu32_gemm = mode;
     c->factored = c->factored_sh = false;
     c->solved = false;
   });

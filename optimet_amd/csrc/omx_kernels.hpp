@@ -116,6 +116,7 @@ struct LuWork {
   int pivot;            // kPivotBlock or kPivotPartial
   int nb = 1;           // systems in the batch (block pivoting only when > 1)
   BatchStrides bs;      // their strides; the pointers above are system 0's
+  bool split = true;             // FP32 factors: bulk updates on the split-bf16 MFMA
   hipEvent_t ev_mark = nullptr;  // when set: recorded on the bulk stream once the
   int mark_col = 0;              // factorisation has passed column mark_col
 };

@@ -22,6 +22,9 @@
 //                  (FP32), four real MFMAs per complex tile product, planar
 //                  re/im operands staged through LDS; panels are taken in
 //                  pairs so that the bulk update runs with K = 2 NB
+//   gemm_split     the FP32 bulk update (K = 2 NB) on the bf16 matrix cores:
+//                  operands split in three bf16 terms, six products per
+//                  real product, FP32 accumulation (FP32-accurate)
 //   trsv_kernel    sync-free blocked triangular solves: workgroup k owns
 //                  block row k, consumes earlier blocks as their flags are
 //                  published and applies its stored diagonal-block inverse
@@ -30,6 +33,7 @@
 // This replaces the ScaLAPACK pzgesv path (scalapack/LinearSystemSolver.hpp:84-91)
 // and Eigen's ColPivHouseholderQR (PreconditionedMatrixSolver.h:58, 75).
 #include <climits>
+#include <type_traits>
 #include <vector>
 
 #include "omx_kernels.hpp"
@@ -839,24 +843,47 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_tile(T *A, size_t lda, in
         for(int b = 0; b < FN; ++b) ci[a][b] = CT<T>::mfma(xi[a], yr[b], ci[a][b]);
     }
   }
+  // C -= the products: the loads of a group of EA block rows are all issued
+  // before its first store (M and N are multiples of 16, so each 16 x 16
+  // block is wholly in or out of range); FP64 takes one block row at a time
+  // to stay within the register budget of four waves per SIMD
+  constexpr int EA = sizeof(R) == 8 ? 1 : FM;
 #pragma unroll
-  for(int a = 0; a < FM; ++a)
+  for(int a0 = 0; a0 < FM; a0 += EA) {
+    R vr[EA][FN][4], vi[EA][FN][4];
 #pragma unroll
-    for(int b = 0; b < FN; ++b) {
-      const int n = bn + wn * WN + b * 16 + (lane & 15);
-      if(n >= N) continue;
-      T *cp = A + (size_t)(col0 + n) * lda + row0;
+    for(int a = 0; a < EA; ++a)
 #pragma unroll
-      for(int v = 0; v < 4; ++v) {
-        const int m = bm + wm * WM + a * 16 + CT<T>::row(lane, v);
-        if(m < M) {
-          T c = cp[m];
-          c.x -= cr[a][b][v];
-          c.y -= ci[a][b][v];
-          cp[m] = c;
+      for(int b = 0; b < FN; ++b) {
+        const int n = bn + wn * WN + b * 16 + (lane & 15), mb = bm + wm * WM + (a0 + a) * 16;
+        const T *cp = A + (size_t)(col0 + n) * lda + row0;
+#pragma unroll
+        for(int v = 0; v < 4; ++v) {
+          vr[a][b][v] = 0;
+          vi[a][b][v] = 0;
+          if(n < N && mb < M) {
+            const T c = cp[mb + CT<T>::row(lane, v)];
+            vr[a][b][v] = c.x;
+            vi[a][b][v] = c.y;
+          }
         }
       }
-    }
+#pragma unroll
+    for(int a = 0; a < EA; ++a)
+#pragma unroll
+      for(int b = 0; b < FN; ++b) {
+        const int n = bn + wn * WN + b * 16 + (lane & 15), mb = bm + wm * WM + (a0 + a) * 16;
+        if(n >= N || mb >= M) continue;
+        T *cp = A + (size_t)(col0 + n) * lda + row0;
+#pragma unroll
+        for(int v = 0; v < 4; ++v) {
+          T c;
+          c.x = vr[a][b][v] - cr[a0 + a][b][v];
+          c.y = vi[a][b][v] - ci[a0 + a][b][v];
+          cp[mb + CT<T>::row(lane, v)] = c;
+        }
+      }
+  }
 }
 
 // Tile configuration of the trailing updates (fastest of a sweep at
@@ -874,6 +901,249 @@ void trailing_update(T *A, size_t lda, int row0, int col0, int k0, int M, int N,
   hipLaunchKernelGGL((gemm_tile<T, U::TM, U::TN, KD, U::WGM, U::WGN, U::BK>),
                      dim3((M + U::TM - 1) / U::TM, (N + U::TN - 1) / U::TN, nb), dim3(64 * U::WGM * U::WGN), 0, st, A,
                      lda, row0, col0, k0, M, N, sA);
+}
+
+// ------------------------------------------------------------------ split-bf16 bulk update
+// The complex-FP32 bulk update C -= A B (K = KD) on the bf16 matrix cores:
+// every FP32 operand x is split in three bf16 terms x = x1 + x2 + x3 (each
+// the round-to-nearest bf16 of what the previous ones leave; x - x1 - x2 - x3
+// is below 2^-24 |x|) and the six products x_i y_j with i + j <= 4 (1-based)
+// are accumulated in FP32 by v_mfma_f32_32x32x16_bf16: the result has the
+// accuracy of an FP32 product (the dropped terms are below 2^-24 |x y|) at
+// six bf16 instructions per real product, against the FP32 MFMA's 1/16 of the
+// bf16 rate.
+//
+// The complex product is a real one over k' = 2 k + part:
+//   D[n][m] = sum_k' X[n][k'] Y[k'][m],   X[n][2k] = Re B(k,n), X[n][2k+1] = Im B(k,n)
+//   Re: Y[2k][m] = Re A(m,k), Y[2k+1][m] = -Im A(m,k)
+//   Im: Y[2k][m] = Im A(m,k), Y[2k+1][m] =  Re A(m,k)
+// X is B's column in its natural (re, im) storage; both Y are formed from the
+// staged A by a sign flip or a swap of the bf16 halves of each 32-bit word.
+// D's lanes run along m, so the epilogue reads and writes C's columns in
+// 128-byte runs.
+//
+// LDS holds the three split planes of a K chunk as 16-byte units (4 complex k
+// of one row of A or one column of B); the global loads of chunk c + 2 are in
+// flight while chunk c is multiplied, and C's tile is fetched during the last
+// chunk.  Workgroups are numbered so that each XCD takes a contiguous range of
+// tiles, walked in groups of GM tile rows (operands reused from its L2).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f32x16 mfma_bf16_32(u32x4 a, u32x4 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                 0, 0);
+}
+
+// f[0..7] -> three words of 8 bf16 each (x1, x2, x3 of every value)
+__device__ inline void split_bf16x3(const float *f, u32x4 &h, u32x4 &m, u32x4 &l) {
+#pragma unroll
+  for(int d = 0; d < 4; ++d) {
+    unsigned hh[2], mm[2], ll[2];
+#pragma unroll
+    for(int e = 0; e < 2; ++e) {
+      const float x = f[2 * d + e];
+      const __bf16 b1 = (__bf16)x;
+      const float r1 = x - (float)b1;
+      const __bf16 b2 = (__bf16)r1;
+      const float r2 = r1 - (float)b2;
+      const __bf16 b3 = (__bf16)r2;
+      hh[e] = __builtin_bit_cast(unsigned short, b1);
+      mm[e] = __builtin_bit_cast(unsigned short, b2);
+      ll[e] = __builtin_bit_cast(unsigned short, b3);
+    }
+    h[d] = hh[0] | (hh[1] << 16);
+    m[d] = mm[0] | (mm[1] << 16);
+    l[d] = ll[0] | (ll[1] << 16);
+  }
+}
+
+template <int TM, int TN, int KD, int WGM, int WGN, int BK, int GM>
+__global__ __launch_bounds__(64 * WGM * WGN) void gemm_split(float2 *A, size_t lda, int row0, int col0, int k0, int M,
+                                                            int N, size_t sA) {
+  A += blockIdx.z * sA;
+  constexpr int NT = 64 * WGM * WGN;
+  constexpr int WM = TM / WGM, WN = TN / WGN, FM = WM / 32, FN = WN / 32;
+  constexpr int KG = BK / 4, NC = KD / BK;
+  constexpr int UA = TM * KG / NT, UB = TN * KG / NT;
+  static_assert(UA * NT == TM * KG && UB * NT == TN * KG, "tile not divisible");
+  static_assert(FM >= 1 && FN >= 1 && BK % 8 == 0 && KD % BK == 0, "tile shape");
+  __shared__ u32x4 Ys[3][KG][TM];
+  __shared__ u32x4 Xs[3][KG][TN];
+  const int gx = (M + TM - 1) / TM, gy = (N + TN - 1) / TN, T = gx * gy;
+  int L = blockIdx.x;
+  const int full = T / 8 * 8;
+  if(L < full) L = (L % 8) * (full / 8) + L / 8;
+  const int per = GM * gy, g0 = (L / per) * GM, gsz = min(gx - g0, GM), r = L % per;
+  const int bm = (g0 + r % gsz) * TM, bn = (r / gsz) * TN;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, wm = wv % WGM, wn = wv / WGM;
+  f32x16 accR[FM][FN], accI[FM][FN];
+#pragma unroll
+  for(int a = 0; a < FM; ++a)
+#pragma unroll
+    for(int b = 0; b < FN; ++b) {
+      accR[a][b] = f32x16{};
+      accI[a][b] = f32x16{};
+    }
+  float ga[2][UA][8], gb[2][UB][8];
+  auto gload = [&](int kc, int sl) {
+#pragma unroll
+    for(int i = 0; i < UA; ++i) {
+      const int u = t + NT * i, m = u % TM, kg = u / TM;
+#pragma unroll
+      for(int j = 0; j < 4; ++j) {
+        ga[sl][i][2 * j] = 0;
+        ga[sl][i][2 * j + 1] = 0;
+        if(bm + m < M) {
+          const float2 v = A[(size_t)(k0 + kc + 4 * kg + j) * lda + row0 + bm + m];
+          ga[sl][i][2 * j] = v.x;
+          ga[sl][i][2 * j + 1] = v.y;
+        }
+      }
+    }
+#pragma unroll
+    for(int i = 0; i < UB; ++i) {
+      const int u = t + NT * i, kg = u % KG, n = u / KG;
+#pragma unroll
+      for(int j = 0; j < 8; ++j) gb[sl][i][j] = 0;
+      if(bn + n < N) {
+        const float4 *p = reinterpret_cast<const float4 *>(A + (size_t)(col0 + bn + n) * lda + k0 + kc + 4 * kg);
+        const float4 p0 = p[0], p1 = p[1];
+        gb[sl][i][0] = p0.x;
+        gb[sl][i][1] = p0.y;
+        gb[sl][i][2] = p0.z;
+        gb[sl][i][3] = p0.w;
+        gb[sl][i][4] = p1.x;
+        gb[sl][i][5] = p1.y;
+        gb[sl][i][6] = p1.z;
+        gb[sl][i][7] = p1.w;
+      }
+    }
+  };
+  // C's tile: lane l, register v of block (a, b) holds D[n][m] with
+  // m = 32 a + l % 32, n = 32 b + 8 (v / 4) + 4 (l / 32) + v % 4; M and N are
+  // multiples of 32, so each 32 x 32 block is wholly in or out of range
+  float vr[FM][FN][16], vi[FM][FN][16];
+  auto cload = [&]() {
+#pragma unroll
+    for(int a = 0; a < FM; ++a)
+#pragma unroll
+      for(int b = 0; b < FN; ++b) {
+        const int mb = bm + wm * WM + 32 * a, nb = bn + wn * WN + 32 * b;
+        const bool ok = mb < M && nb < N;
+#pragma unroll
+        for(int v = 0; v < 16; ++v) {
+          vr[a][b][v] = 0;
+          vi[a][b][v] = 0;
+          if(ok) {
+            const int n = nb + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4), m = mb + (lane & 31);
+            const float2 c = A[(size_t)(col0 + n) * lda + row0 + m];
+            vr[a][b][v] = c.x;
+            vi[a][b][v] = c.y;
+          }
+        }
+      }
+  };
+  gload(0, 0);
+  if(NC > 1) gload(BK, 1);
+#pragma unroll
+  for(int c = 0; c < NC; ++c) {
+    const int sl = c & 1;
+    __syncthreads();  // the previous chunk has been read out of LDS
+#pragma unroll
+    for(int i = 0; i < UA; ++i) {
+      const int u = t + NT * i, m = u % TM, kg = u / TM;
+      u32x4 h, md, lo;
+      split_bf16x3(ga[sl][i], h, md, lo);
+      Ys[0][kg][m] = h;
+      Ys[1][kg][m] = md;
+      Ys[2][kg][m] = lo;
+    }
+#pragma unroll
+    for(int i = 0; i < UB; ++i) {
+      const int u = t + NT * i, kg = u % KG, n = u / KG;
+      u32x4 h, md, lo;
+      split_bf16x3(gb[sl][i], h, md, lo);
+      Xs[0][kg][n] = h;
+      Xs[1][kg][n] = md;
+      Xs[2][kg][n] = lo;
+    }
+    __syncthreads();
+    if(c + 2 < NC) gload((c + 2) * BK, sl);
+    if(c + 1 == NC) cload();
+#pragma unroll
+    for(int ks = 0; ks < BK / 8; ++ks) {
+      // lane l feeds rows/columns l % 32 at k' = 16 ks + 8 (l / 32) .. + 7
+      const int kg = 2 * ks + (lane >> 5), li = lane & 31;
+      u32x4 x[3][FN], yr[3][FM], yi[3][FM];
+#pragma unroll
+      for(int s = 0; s < 3; ++s) {
+#pragma unroll
+        for(int b = 0; b < FN; ++b) x[s][b] = Xs[s][kg][wn * WN + 32 * b + li];
+#pragma unroll
+        for(int a = 0; a < FM; ++a) {
+          const u32x4 y = Ys[s][kg][wm * WM + 32 * a + li];
+#pragma unroll
+          for(int d = 0; d < 4; ++d) {
+            yr[s][a][d] = y[d] ^ 0x80000000u;
+            yi[s][a][d] = (y[d] >> 16) | (y[d] << 16);
+          }
+        }
+      }
+      // the small products first
+      constexpr int SY[6] = {2, 1, 0, 1, 0, 0}, SX[6] = {0, 1, 2, 0, 1, 0};
+#pragma unroll
+      for(int p = 0; p < 6; ++p)
+#pragma unroll
+        for(int a = 0; a < FM; ++a)
+#pragma unroll
+          for(int b = 0; b < FN; ++b) {
+            accR[a][b] = mfma_bf16_32(x[SX[p]][b], yr[SY[p]][a], accR[a][b]);
+            accI[a][b] = mfma_bf16_32(x[SX[p]][b], yi[SY[p]][a], accI[a][b]);
+          }
+    }
+  }
+#pragma unroll
+  for(int a = 0; a < FM; ++a)
+#pragma unroll
+    for(int b = 0; b < FN; ++b) {
+      const int mb = bm + wm * WM + 32 * a, nb = bn + wn * WN + 32 * b;
+      if(mb >= M || nb >= N) continue;
+#pragma unroll
+      for(int v = 0; v < 16; ++v) {
+        const int n = nb + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4), m = mb + (lane & 31);
+        float2 c;
+        c.x = vr[a][b][v] - accR[a][b][v];
+        c.y = vi[a][b][v] - accI[a][b][v];
+        A[(size_t)(col0 + n) * lda + row0 + m] = c;
+      }
+    }
+}
+
+// Tile of the split-bf16 update (fastest of a sweep at M = N = 10112,
+// K = 128: 145 TFLOP/s complex, against 87 for gemm_tile<float2> on the same
+// random data).
+struct SplitTile {
+  static constexpr int TM = 128, TN = 128, WGM = 4, WGN = 2, BK = 16, GM = 8;
+};
+
+// The bulk update with K = 2 NB: split-bf16 for FP32 factors when `split`,
+// gemm_tile otherwise.  M and N are multiples of 64.
+template <typename T>
+void bulk_update(T *A, size_t lda, int row0, int col0, int k0, int M, int N, hipStream_t st, int nb, size_t sA,
+                 bool split) {
+  if(M <= 0 || N <= 0) return;
+  if constexpr(std::is_same<T, float2>::value) {
+    if(split) {
+      using U = SplitTile;
+      const int tiles = ((M + U::TM - 1) / U::TM) * ((N + U::TN - 1) / U::TN);
+      hipLaunchKernelGGL((gemm_split<U::TM, U::TN, 2 * kNB, U::WGM, U::WGN, U::BK, U::GM>), dim3(tiles, 1, nb),
+                         dim3(64 * U::WGM * U::WGN), 0, st, A, lda, row0, col0, k0, M, N, sA);
+      return;
+    }
+  }
+  trailing_update<T, 2 * kNB>(A, lda, row0, col0, k0, M, N, st, nb, sA);
 }
 
 // ------------------------------------------------------------------ driver
@@ -959,9 +1229,9 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
       if(rec) (void)hipEventRecord(pe[2], st);
       if(two && restf > 0) {
         const int nn = restf < 2 * kNB ? restf : 2 * kNB;
-        trailing_update<T, 2 * kNB>(A, lda, far, far, p0, restf, nn, st, nb, bs.a);
+        bulk_update<T>(A, lda, far, far, p0, restf, nn, st, nb, bs.a, w.split);
         (void)hipEventRecord(w.ev_next, st);
-        trailing_update<T, 2 * kNB>(A, lda, far, far + nn, p0, restf, restf - nn, st, nb, bs.a);
+        bulk_update<T>(A, lda, far, far + nn, p0, restf, restf - nn, st, nb, bs.a, w.split);
         (void)hipEventRecord(w.ev_bulk, st);
         if(rec) {
           prof->gemm_flop += 8.0 * (double)restf * restf * (2 * kNB) * nb;
@@ -1040,9 +1310,9 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
     }
     if(p1 < Np && rest1 > 0) {
       const int Jn = p1 + kNB;
-      trailing_update<T, 2 * kNB>(A, lda, Jn, Jn, p0, rest1, kNB, st);
+      bulk_update<T>(A, lda, Jn, Jn, p0, rest1, kNB, st, 1, 0, w.split);
       pivoting(Jn);
-      trailing_update<T, 2 * kNB>(A, lda, Jn, Jn + kNB, p0, rest1, rest1 - kNB, st);
+      bulk_update<T>(A, lda, Jn, Jn + kNB, p0, rest1, rest1 - kNB, st, 1, 0, w.split);
       if(rec) {
         prof->gemm_flop += 8.0 * (double)rest1 * rest1 * (2 * kNB);
         prof->gemm_launches += rest1 > kNB ? 2 : 1;

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "liboptimet_mi355x.so")
 LU64, LU32_IR, GMRES64 = 0, 1, 2
 PIVOT_BLOCK, PIVOT_PARTIAL = 0, 1
 PIVOTS = {"block": PIVOT_BLOCK, "partial": PIVOT_PARTIAL}
+GEMMS = {"bf16x3": 0, "fp32": 1}
 KINDS = {"lu64": LU64, "lu32ir": LU32_IR, "gmres64": GMRES64}
 
 ERRORS = {-1: "argument", -2: "device", -3: "memory", -4: "singular", -5: "state", -6: "no convergence",
@@ -26,7 +27,7 @@ ERRORS = {-1: "argument", -2: "device", -3: "memory", -4: "singular", -5: "state
 EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "omx_set_excitation",
            "omx_set_solver", "omx_set_cg_tables", "omx_build_cg_tables", "omx_update", "omx_solve",
            "omx_cross_sections", "omx_scattering_size", "omx_get_stats", "omx_read_matrix", "omx_read_vector",
-           "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_solve_sweep", "omx_mem_info", "omx_trim"]
+           "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_set_lu32_gemm", "omx_solve_sweep", "omx_mem_info", "omx_trim"]
 
 
 class OmxError(RuntimeError):
@@ -81,6 +82,7 @@ def load_library(path=LIB_PATH):
     L.omx_dense_solve.argtypes = [P, I, D, D, D, I]
     L.omx_set_profile.argtypes = [P, I]
     L.omx_set_pivoting.argtypes = [P, I]
+    L.omx_set_lu32_gemm.argtypes = [P, I]
     L.omx_solve_sweep.argtypes = [P, I, D, D, D, I, D, D, D, D, D, ctypes.POINTER(ctypes.c_int)]
     SZ = ctypes.POINTER(ctypes.c_size_t)
     L.omx_mem_info.argtypes = [P, SZ, SZ, SZ]
@@ -143,6 +145,12 @@ class Context:
     def set_pivoting(self, mode):
         """"block" (default) or "partial" (OMX_PIVOT_*)."""
         _check(self.L.omx_set_pivoting(self.h, PIVOTS[mode]))
+
+    def set_lu32_gemm(self, mode):
+        """"bf16x3" (default: FP32 operands split in three bf16 terms on the
+        bf16 matrix cores) or "fp32" (the FP32 matrix cores): the products of
+        the mixed-precision factorisation's bulk updates (OMX_GEMM_*)."""
+        _check(self.L.omx_set_lu32_gemm(self.h, GEMMS[mode]))
 
     def dense_solve(self, A, b, kind="lu64"):
         A = np.asfortranarray(A, dtype=np.complex128)

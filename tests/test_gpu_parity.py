@@ -132,6 +132,44 @@ def test_dense_solve_random(ctx, N, kind, tol):
     assert rel(x, xr) < tol
 
 
+@pytest.mark.parametrize("N", [300, 700, 1500])
+def test_lu32_gemm_modes(ctx, N):
+    """The mixed path's bulk updates on the bf16 matrix cores (three-term
+    splits, six products) and on the FP32 ones: the same solution, and an
+    FP32-accurate factorisation -- on a matrix with no diagonal boost (so the
+    refinement's contraction is set by the factors' accuracy) the refinement
+    takes as many iterations as with FP32 products, give or take one."""
+    rng = np.random.default_rng(100 + N)
+    A = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    xr = np.linalg.solve(A, b)
+    its = {}
+    try:
+        for mode in ("fp32", "bf16x3"):
+            ctx.set_lu32_gemm(mode)
+            x = ctx.dense_solve(A, b, "lu32ir")
+            its[mode] = ctx.stats()["iterations"]
+            assert rel(x, xr) < 1e-9, mode
+    finally:
+        ctx.set_lu32_gemm("bf16x3")
+    assert abs(its["bf16x3"] - its["fp32"]) <= 1, its
+
+
+def test_lu32_gemm_modes_cluster(ctx, ora):
+    """N = 768 cluster, mixed path with either product mode, against the oracle."""
+    c = drude_cluster(8, 6, 750e-9, 40, 80, 800, 10, seed=21, sh=True)
+    ref = ora.solve_all(c)
+    try:
+        for mode in ("fp32", "bf16x3"):
+            ctx.set_lu32_gemm(mode)
+            s = make(ctx, c, "lu32ir")
+            out = s.solve()
+            assert rel(out["x_sca"], ref["x_sca"]) < MIXED_TOL, mode
+            assert rel(out["x_sca_sh"], ref["x_sca_sh"]) < 100 * MIXED_TOL, mode
+    finally:
+        ctx.set_lu32_gemm("bf16x3")
+
+
 def test_dense_solve_needs_pivoting(ctx):
     """Zero leading diagonal: only a pivoting LU survives."""
     rng = np.random.default_rng(3)
