@@ -1339,16 +1339,22 @@ __global__ void gather_kernel(const cplx *b, const int *perm, cplx *y, int Np, B
 
 // Sync-free blocked triangular solve.  Workgroup K owns SB consecutive
 // 64-row blocks (a super-block).  Far phase: it consumes the blocks x_b of
-// earlier super-blocks as their flags are published (agent-scope acquire,
-// bounded spin) and accumulates L[super-block rows, b] x_b, with the next
-// 64 x 16 tile slice of each thread prefetched into registers while the
-// current one is multiplied.  Near phase: the super-block's own blocks are
-// solved in sequence inside the workgroup -- the contributions of the blocks
-// just solved, then the stored inverse of the diagonal block -- so only every
-// SB-th link of the chain crosses workgroups.  With PRE the near phase's tiles
-// (diagonal-block inverses and the intra-super-block L tiles) are loaded into
-// registers at kernel start.  Thread t handles row r = t % 64 and columns
-// 16 (t / 64) .. 16 (t / 64) + 15 of every tile.
+// earlier super-blocks as their flags are published and accumulates
+// L[super-block rows, b] x_b, with the next 64 x 16 tile slice of each thread
+// prefetched into registers while the current one is multiplied.  Near phase: the
+// super-block's own blocks are solved in sequence inside the workgroup -- the
+// contributions of the blocks just solved, then the stored inverse of the
+// diagonal block -- so only every SB-th link of the chain crosses
+// workgroups.  With PRE the near phase's tiles (diagonal-block inverses and
+// the intra-super-block L tiles) are loaded into registers at kernel start
+// (PRE = 1) or after the far phase (PRE = 2).  Thread t handles row
+// r = t % 64 and columns 16 (t / 64) .. 16 (t / 64) + 15 of every tile.
+//
+// Publication: a solved block is written with device-coherent (agent-scope)
+// stores, the workgroup waits for their completion, and one thread then
+// stores the block's flag, also device-coherent; consumers poll the flags
+// and read x with device-coherent loads, so neither side writes back or
+// invalidates the L2.
 template <typename T, bool LOWER, int SB, int PRE>
 __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size_t lda, int Np,
                                                    const T *__restrict__ dinv, cplx *x, int *flags, int epoch,
@@ -1358,7 +1364,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
   A += blockIdx.z * bs.a;
   dinv += blockIdx.z * bs.inv;
   x += blockIdx.z * bs.vec;
-  flags += (size_t)blockIdx.z * bs.flags;  // intra-super-block tiles
+  flags += (size_t)blockIdx.z * bs.flags;
   const int nblk = Np / NB, nsb = (nblk + SB - 1) / SB;
   const int K = LOWER ? blockIdx.x : nsb - 1 - blockIdx.x;
   const int kb0 = K * SB, nb = min(SB, nblk - kb0);
@@ -1367,9 +1373,14 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
   constexpr int CH = 8;  // far blocks taken per round
   __shared__ cplx xs[CH * NB];
   __shared__ cplx part[4][NB];
+  __shared__ cplx part2[SB][4][NB];
   __shared__ int nready_s;
   __shared__ cplx rhs[SB][NB];
   __shared__ cplx xsb[SB][NB];
+  // right-hand sides of the super-block (only this workgroup writes them)
+  cplx b0[SB];
+#pragma unroll
+  for(int s = 0; s < SB; ++s) b0[s] = (t < NB && s < nb) ? x[(size_t)(kb0 + s) * NB + t] : mk(0.0);
   // the thread's 16 entries of a 64 x 64 tile at base (column-major, leading
   // dimension ld): a wave-uniform column address plus the lane's row offset
   auto tile = [&](const T *base, size_t ld, T (&dst)[16]) {
@@ -1398,13 +1409,13 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
       }
   };
   if(PRE == 1) preload();
+  const int nfar = LOWER ? kb0 : nblk - (kb0 + nb);
+  auto fblk = [&](int j) { return LOWER ? j : nblk - 1 - j; };  // j-th far block in order of publication
   // ---- far phase
   cplx acc[SB];
 #pragma unroll
   for(int s = 0; s < SB; ++s) acc[s] = mk(0.0);
-  const int nfar = LOWER ? kb0 : nblk - (kb0 + nb);
   T cur[16], nxt[16];
-  auto fblk = [&](int j) { return LOWER ? j : nblk - 1 - j; };  // j-th far block in order of publication
   if(nfar > 0) tile(a_base(kb0, fblk(0)), lda, cur);
   for(int j0 = 0; j0 < nfar;) {
     // how many of the next CH far blocks are published (wave 0, one flag per lane)
@@ -1413,7 +1424,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
       int nr = 0;
       while(true) {
         const bool ok = t < CH && j0 + t < nfar &&
-                        __hip_atomic_load(&flags[fblk(j0 + t)], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+                        __hip_atomic_load(&flags[fblk(j0 + t)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
         const unsigned long long m = __ballot(ok);
         nr = __builtin_ctzll(~m);  // leading run of published blocks
         if(nr > 0 || __hip_atomic_load(&status[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
@@ -1427,8 +1438,8 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
     }
     __syncthreads();
     const int nr = nready_s;
-    // those blocks of x were published with a release before their flags:
-    // read them with agent-scope atomic loads instead of invalidating caches
+    // those blocks of x were stored device-coherently before their flags:
+    // read them with device-coherent loads
     for(int idx = t; idx < nr * NB; idx += 256) {
       const unsigned long long *src =
           reinterpret_cast<const unsigned long long *>(x + (size_t)fblk(j0 + idx / NB) * NB + (idx % NB));
@@ -1463,14 +1474,14 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
   if(PRE == 2) preload();
   // right-hand sides of the super-block's blocks minus the far contributions
 #pragma unroll
-  for(int s = 0; s < SB; ++s) {
-    if(s < nb) {
-      part[cq][r] = acc[s];
-      __syncthreads();
-      if(t < NB) rhs[s][t] = x[(size_t)(kb0 + s) * NB + t] - (part[0][t] + part[1][t] + part[2][t] + part[3][t]);
-      __syncthreads();
-    }
+  for(int s = 0; s < SB; ++s) part2[s][cq][r] = acc[s];
+  __syncthreads();
+  if(t < NB) {
+#pragma unroll
+    for(int s = 0; s < SB; ++s)
+      if(s < nb) rhs[s][t] = b0[s] - (part2[s][0][t] + part2[s][1][t] + part2[s][2][t] + part2[s][3][t]);
   }
+  __syncthreads();
   // ---- near phase: the super-block's blocks in order
 #pragma unroll
   for(int i = 0; i < SB; ++i) {
@@ -1480,12 +1491,14 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
       // contributions of the blocks of this super-block solved before
       cplx a = mk(0.0);
       int q = 0;
+      bool any = false;
 #pragma unroll
       for(int ss = 0; ss < SB; ++ss)
 #pragma unroll
         for(int s2 = 0; s2 < SB; ++s2) {
           if(LOWER ? (s2 < ss) : (s2 > ss)) {
             if(ss == s && s2 < nb) {
+              any = true;
               T lt[16];
               if(PRE) {
 #pragma unroll
@@ -1503,9 +1516,13 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
             ++q;
           }
         }
-      part[cq][r] = a;
-      __syncthreads();
-      if(t < NB) xs[t] = rhs[s][t] - (part[0][t] + part[1][t] + part[2][t] + part[3][t]);
+      if(any) {
+        part[cq][r] = a;
+        __syncthreads();
+        if(t < NB) xs[t] = rhs[s][t] - (part[0][t] + part[1][t] + part[2][t] + part[3][t]);
+      } else {
+        if(t < NB) xs[t] = rhs[s][t];
+      }
       __syncthreads();
       // x_k = (diagonal block)^{-1} r
       T m[16];
@@ -1527,11 +1544,16 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
       if(t < NB) {
         const cplx xk = part[0][t] + part[1][t] + part[2][t] + part[3][t];
         xsb[s][t] = xk;
-        x[(size_t)k * NB + t] = xk;
+        unsigned long long *dst = reinterpret_cast<unsigned long long *>(x + (size_t)k * NB + t);
+        __hip_atomic_store(dst, (unsigned long long)__double_as_longlong(xk.x), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(dst + 1, (unsigned long long)__double_as_longlong(xk.y), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      // the stores above have completed (device-coherent) before the flag
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __syncthreads();
-      if(t == 0) __hip_atomic_store(&flags[k], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if(t == 0) __hip_atomic_store(&flags[k], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
