@@ -158,7 +158,7 @@ void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags
   const int Np = c->Np;
   const int gmax = (Np + omx::kPanelRows - 1) / omx::kPanelRows + 1;
   piv.ensure(sizeof(int) * Np, &c->total);
-  dinv.ensure(sizeof(cplx) * 2 * omx::kNB * (size_t)Np, &c->total);
+  dinv.ensure(sizeof(cplx) * omx::dinv_elems(Np), &c->total);
   perm.ensure(sizeof(int) * Np, &c->total);
   flags.ensure(sizeof(int) * (Np / omx::kNB + 1), &c->total);
   // tournament scratch: one half per system, as S and V may be factored
@@ -749,7 +749,7 @@ void sweep_alloc(omx_context *c, int nb, bool f64, bool sh) {
   w.cross.ensure(sizeof(double) * 4 * P * nb, T);
   for(DBuf *b : {&w.perm, &w.perm2}) b->ensure(sizeof(int) * Np * nb, T);
   for(DBuf *b : {&w.flags, &w.flags2}) b->ensure(sizeof(int) * (Np / omx::kNB + 1) * nb, T);
-  for(DBuf *b : {&w.dinv, &w.dinv2}) b->ensure(sizeof(cplx) * 2 * omx::kNB * Np * nb, T);
+  for(DBuf *b : {&w.dinv, &w.dinv2}) b->ensure(sizeof(cplx) * omx::dinv_elems(Np) * nb, T);
   w.map.ensure(sizeof(int) * 3 * (Np + 2 * omx::kNB) * nb, T);
   w.status.ensure(sizeof(int) * 4 * nb, T);
   w.basis.ensure(sizeof(cplx) * Np * (m + 1) * nb, T);
@@ -786,7 +786,7 @@ omx::LuWork sweep_lu(omx_context *c, int nb, bool second) {
   lw.split = c->lu32_gemm == OMX_GEMM_BF16X3;
   lw.nb = nb;
   lw.bs.a = (size_t)Np * Np;
-  lw.bs.inv = (size_t)2 * omx::kNB * Np;
+  lw.bs.inv = omx::dinv_elems(Np);
   lw.bs.vec = (size_t)Np;
   lw.bs.perm = Np;
   lw.bs.map = ms;

@@ -41,7 +41,7 @@ def footprint(cluster, batch, kind):
     Np = -(-cluster.N // 64) * 64
     per_matrix = Np * Np * (24 if kind == "lu32ir" else 32)
     mats = 2 if cluster.sh else 1
-    vectors = Np * (16 * 16 + 2 * 16 * 2 * 64 + 16 * 41 + 16 * 32)
+    vectors = Np * (16 * 16 + 2 * 16 * 3 * 64 + 16 * 41 + 16 * 32)
     return batch * (mats * per_matrix + vectors)
 
 
