@@ -1235,22 +1235,33 @@ int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, do
       // V's factorisation starts once S's has passed a fraction of its
       // columns (by then S's trailing updates no longer fill the device),
       // and overlaps the rest of it and the fundamental refinement
+      // (V's work is enqueued right at the mark, in the middle of S's
+      // enqueueing: a wait on the mark event enqueued only after all of S's
+      // work was seen to hold V back until S's end)
       const int mark = (int)(c->sh_overlap * c->Np);
+      auto start_v = [](void *arg) {
+        omx_context *cc = static_cast<omx_context *>(arg);
+        HIPCHK(hipStreamWaitEvent(cc->st2, cc->xev[0], 0));
+        factor_enqueue(cc, true, cc->pivot);
+      };
       if(mark > 0 && !c->factored) {
         c->lu.ev_mark = c->xev[0];
         c->lu.mark_col = mark;
+        c->lu.on_mark = start_v;
+        c->lu.mark_arg = c;
         try {
           factor_enqueue(c, false, c->pivot);
         } catch(...) {
           c->lu.ev_mark = nullptr;
+          c->lu.on_mark = nullptr;
           throw;
         }
         c->lu.ev_mark = nullptr;
+        c->lu.on_mark = nullptr;
       } else {
         HIPCHK(hipEventRecord(c->xev[0], c->st));
+        start_v(c);
       }
-      HIPCHK(hipStreamWaitEvent(c->st2, c->xev[0], 0));
-      factor_enqueue(c, true, c->pivot);
     }
     // X_sca = S^{-1} Q (PreconditionedMatrixSolver.h:58)
     solve_system(c, false, c->d_Q.as<cplx>(), c->d_x.as<cplx>(), &c->stats.iterations, &c->stats.residual);

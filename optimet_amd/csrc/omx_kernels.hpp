@@ -121,7 +121,9 @@ struct LuWork {
   BatchStrides bs;      // their strides; the pointers above are system 0's
   bool split = true;             // FP32 factors: bulk updates on the split-bf16 MFMA
   hipEvent_t ev_mark = nullptr;  // when set: recorded on the bulk stream once the
-  int mark_col = 0;              // factorisation has passed column mark_col
+  int mark_col = 0;              // factorisation has passed column mark_col, and
+  void (*on_mark)(void *) = nullptr;  // then on_mark(mark_arg) is called (on the host,
+  void *mark_arg = nullptr;           // in the middle of the enqueueing)
 };
 // Elements of T in a factorisation's dinv: 2 NB^2 elements of T per panel,
 // then from element 2 NB Np on, 2 NB^2 complex FP64 values per pair of panels.

@@ -1249,6 +1249,12 @@ __global__ __launch_bounds__(256) void sb_inverse(const T *A, size_t lda, int Np
 //
 // Tournament (partial) pivoting: one stream for the panel sequence, sb for
 // the tournaments; every interchange is applied at once to all columns.
+// Progress mark of a factorisation (see LuWork::ev_mark).
+inline void mark(LuWork &w, hipStream_t st) {
+  (void)hipEventRecord(w.ev_mark, st);
+  if(w.on_mark) w.on_mark(w.mark_arg);
+}
+
 template <typename T>
 int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof) {
   if(Np % kNB) return -1;
@@ -1328,9 +1334,9 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
         (void)hipEventRecord(pe[3], st);
         prof->used = pi + 1;
       }
-      if(w.ev_mark && far >= w.mark_col && far - 2 * kNB < w.mark_col) (void)hipEventRecord(w.ev_mark, st);
+      if(w.ev_mark && far >= w.mark_col && far - 2 * kNB < w.mark_col) mark(w, st);
     }
-    if(w.ev_mark && Np < w.mark_col) (void)hipEventRecord(w.ev_mark, st);
+    if(w.ev_mark && Np < w.mark_col) mark(w, st);
     if(Np > kNB)
       hipLaunchKernelGGL((laswp_left<T>), dim3((Np - kNB) / 8, Np / kNB - 1, nb), dim3(256), 0, st, A, lda, w.mdst,
                          w.msrc, w.mcnt, bs);
@@ -1410,9 +1416,9 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
       (void)hipEventRecord(pe[3], st);
       prof->used = pi + 1;
     }
-    if(w.ev_mark && p1 + kNB >= w.mark_col && p0 < w.mark_col) (void)hipEventRecord(w.ev_mark, st);
+    if(w.ev_mark && p1 + kNB >= w.mark_col && p0 < w.mark_col) mark(w, st);
   }
-  if(w.ev_mark && Np < w.mark_col) (void)hipEventRecord(w.ev_mark, st);
+  if(w.ev_mark && Np < w.mark_col) mark(w, st);
   if(Np >= 2 * kNB)
     hipLaunchKernelGGL((sb_inverse<T>), dim3(Np / kNB / 2, 2, 1), dim3(256), 0, st, A, lda, Np, dinv, BatchStrides{});
   return 0;
