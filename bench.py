@@ -238,8 +238,13 @@ def config5_main(args):
     out = {}
     for kind in (("lu32ir",) if args.no_fp64 else ("lu32ir", "lu64")):
         srv = Server(slots=args.slots, kind=kind)
+
+        def progress(r, kind=kind):
+            print(f"[{kind}] request {r.id} done at {r.finished:.1f} s (latency {r.latency:.1f} s"
+                  f"{', error' if r.error else ''})", file=sys.stderr, flush=True)
+
         try:
-            res = srv.run(reqs, realtime=True, deadline=args.deadline)
+            res = srv.run(reqs, realtime=True, deadline=args.deadline, on_done=progress)
         finally:
             srv.close()
         summ = summarize(res, nl)

@@ -318,8 +318,10 @@ struct Gmres {
       double beta = nrm(r);
       *res = beta / bn;
       if(*res <= tol || it >= maxit || tiny_step) break;
-      // stagnation at the FP64 floor: two restart cycles without halving
-      if(prev > 0.0 && beta > 0.5 * prev) {
+      // stagnation at the FP64 floor: two restart cycles without halving, once
+      // the residual is down to 1e-10 (further up a slow phase of the
+      // restarted iteration is waited out, up to maxit)
+      if(prev > 0.0 && beta > 0.5 * prev && *res < 1e-10) {
         if(++stall >= 2) break;
       } else {
         stall = 0;

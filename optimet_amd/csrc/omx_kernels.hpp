@@ -108,10 +108,7 @@ struct LuWork {
   int *status;          // [0]: singular pivot column+1, [1]: wait timeout
   int *flags;           // Np / NB triangular-solve flags
   void *wlu;            // NB x NB complex: the tournament's selected-block factors
-  void *dinv;           // per panel [L11^{-1} | U11^{-1}], 2 NB^2 elements of T, column-major,
-                        // then per pair of panels the off-diagonal blocks of the inverses
-                        // of the 2 x 2 block super-blocks (lower, upper; complex FP64):
-                        // dinv_elems(Np) elements of T in all
+  void *dinv;           // per panel [L11^{-1} | U11^{-1}], 2 NB^2 elements of T, column-major
   hipStream_t aux;      // look-ahead stream of the panel pivoting
   hipEvent_t ev_next, ev_piv, ev_bulk, ev_pan;
   int gmax;
@@ -125,9 +122,8 @@ struct LuWork {
   void (*on_mark)(void *) = nullptr;  // then on_mark(mark_arg) is called (on the host,
   void *mark_arg = nullptr;           // in the middle of the enqueueing)
 };
-// Elements of T in a factorisation's dinv: 2 NB^2 elements of T per panel,
-// then from element 2 NB Np on, 2 NB^2 complex FP64 values per pair of panels.
-inline size_t dinv_elems(int Np) { return 4 * (size_t)64 * Np + 64 * 64; }
+// Elements of T in a factorisation's dinv: 2 NB^2 per panel.
+inline size_t dinv_elems(int Np) { return 2 * (size_t)64 * Np; }
 
 // Pivot search of a panel: among the rows of its diagonal block only
 // (followed by iterative refinement), or over the whole column (tournament).

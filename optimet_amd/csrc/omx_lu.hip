@@ -1167,71 +1167,6 @@ void bulk_update(T *A, size_t lda, int row0, int col0, int k0, int M, int N, hip
   trailing_update<T, 2 * kNB>(A, lda, row0, col0, k0, M, N, st, nb, sA);
 }
 
-// ------------------------------------------------------------------ super-block inverses
-// The triangular solves take the diagonal blocks in pairs (super-blocks of
-// 128 rows).  The inverse of a pair is [L00^{-1} 0; X L11^{-1}] (lower) or
-// [U00^{-1} X; 0 U11^{-1}] (upper) with
-//   lower  X = -L11^{-1} L10 L00^{-1}      upper  X = -U00^{-1} U01 U11^{-1}
-// computed here from the blocks' stored inverses once per factorisation
-// (workgroup: blockIdx.x = pair, blockIdx.y = 0 lower / 1 upper) and written
-// in complex FP64 (also for FP32 factors: rounding X to FP32 costs the
-// refinement an iteration) from element 2 NB Np of dinv on, at
-// (2 pair + y) NB^2, column-major.
-template <typename T>
-__global__ __launch_bounds__(256) void sb_inverse(const T *A, size_t lda, int Np, T *dinv, BatchStrides bs) {
-  constexpr int NB = kNB;
-  A += blockIdx.z * bs.a;
-  dinv += blockIdx.z * bs.inv;
-  const int K = blockIdx.x, up = blockIdx.y, k0 = 2 * K, k1 = 2 * K + 1;
-  const T *P = dinv + (size_t)(up ? k0 : k1) * 2 * NB * NB + (up ? NB * NB : 0);
-  const T *R = dinv + (size_t)(up ? k1 : k0) * 2 * NB * NB + (up ? NB * NB : 0);
-  const T *Q = up ? A + (size_t)(k1 * NB) * lda + (size_t)k0 * NB : A + (size_t)(k0 * NB) * lda + (size_t)k1 * NB;
-  cplx *X = reinterpret_cast<cplx *>(dinv + 2 * (size_t)NB * Np) + (size_t)(2 * K + up) * NB * NB;
-  // products in FP64: the two inverses may be large where X is not
-  __shared__ T s1[NB * NB];
-  __shared__ cplx s2[NB * NB];
-  const int t = threadIdx.x, i = t & 63, j0 = (t >> 6) * 16;
-#pragma unroll 4
-  for(int idx = t; idx < NB * NB; idx += 256) {
-    s1[idx] = Q[(size_t)(idx / NB) * lda + idx % NB];
-    const T v = R[idx];
-    s2[idx] = mk((double)v.x, (double)v.y);
-  }
-  __syncthreads();
-  // rows i, columns j0 .. j0 + 15 of Q R
-  double cr[16], ci[16];
-#pragma unroll
-  for(int c = 0; c < 16; ++c) cr[c] = ci[c] = 0;
-  for(int k = 0; k < NB; ++k) {
-    const T a = s1[k * NB + i];
-#pragma unroll
-    for(int c = 0; c < 16; ++c) {
-      const cplx b = s2[(j0 + c) * NB + k];
-      cr[c] += (double)a.x * b.x - (double)a.y * b.y;
-      ci[c] += (double)a.x * b.y + (double)a.y * b.x;
-    }
-  }
-  __syncthreads();
-#pragma unroll 4
-  for(int idx = t; idx < NB * NB; idx += 256) s1[idx] = P[idx];
-#pragma unroll
-  for(int c = 0; c < 16; ++c) s2[(j0 + c) * NB + i] = mk(cr[c], ci[c]);
-  __syncthreads();
-#pragma unroll
-  for(int c = 0; c < 16; ++c) cr[c] = ci[c] = 0;
-  for(int k = 0; k < NB; ++k) {
-    const T a = s1[k * NB + i];
-#pragma unroll
-    for(int c = 0; c < 16; ++c) {
-      const cplx b = s2[(j0 + c) * NB + k];
-      cr[c] += (double)a.x * b.x - (double)a.y * b.y;
-      ci[c] += (double)a.x * b.y + (double)a.y * b.x;
-    }
-  }
-#pragma unroll
-  for(int c = 0; c < 16; ++c) X[(size_t)(j0 + c) * NB + i] = mk(-cr[c], -ci[c]);
-}
-
 // ------------------------------------------------------------------ driver
 // Panels of NB columns are taken in pairs (p0 = J, p1 = J + NB) so that the
 // bulk of the trailing update runs with K = 2 NB.
@@ -1340,8 +1275,6 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
     if(Np > kNB)
       hipLaunchKernelGGL((laswp_left<T>), dim3((Np - kNB) / 8, Np / kNB - 1, nb), dim3(256), 0, st, A, lda, w.mdst,
                          w.msrc, w.mcnt, bs);
-    if(Np >= 2 * kNB)
-      hipLaunchKernelGGL((sb_inverse<T>), dim3(Np / kNB / 2, 2, nb), dim3(256), 0, st, A, lda, Np, dinv, bs);
     return 0;
   }
 
@@ -1419,8 +1352,6 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
     if(w.ev_mark && p1 + kNB >= w.mark_col && p0 < w.mark_col) mark(w, st);
   }
   if(w.ev_mark && Np < w.mark_col) mark(w, st);
-  if(Np >= 2 * kNB)
-    hipLaunchKernelGGL((sb_inverse<T>), dim3(Np / kNB / 2, 2, 1), dim3(256), 0, st, A, lda, Np, dinv, BatchStrides{});
   return 0;
 }
 
@@ -1433,18 +1364,18 @@ __global__ void gather_kernel(const cplx *b, const int *perm, cplx *y, int Np, B
   if(i < Np) y[i] = b[perm[i]];
 }
 
-// Sync-free blocked triangular solve.  Workgroup K owns SB = 2 consecutive
+// Sync-free blocked triangular solve.  Workgroup K owns SB consecutive
 // 64-row blocks (a super-block).  Far phase: it consumes the blocks x_b of
 // earlier super-blocks as their flags are published and accumulates
 // L[super-block rows, b] x_b, with the next 64 x 16 tile slice of each thread
-// prefetched into registers while the current one is multiplied.  Near phase:
-// both blocks in one product with the stored inverse of the 128 x 128
-// diagonal super-block (the blocks' inverses and the off-diagonal block X
-// written by sb_inverse), so the chain of dependent steps has one link per
-// super-block.  With PRE the near phase's three tiles are loaded into
-// registers at kernel start (PRE = 1) or after the far phase (PRE = 2).
-// Thread t handles row r = t % 64 and columns 16 (t / 64) .. 16 (t / 64) + 15
-// of every tile.
+// prefetched into registers while the current one is multiplied.  Near phase: the
+// super-block's own blocks are solved in sequence inside the workgroup -- the
+// contributions of the blocks just solved, then the stored inverse of the
+// diagonal block -- so only every SB-th link of the chain crosses
+// workgroups.  With PRE the near phase's tiles (diagonal-block inverses and
+// the intra-super-block L tiles) are loaded into registers at kernel start
+// (PRE = 1) or after the far phase (PRE = 2).  Thread t handles row
+// r = t % 64 and columns 16 (t / 64) .. 16 (t / 64) + 15 of every tile.
 //
 // Publication: a solved block is written with device-coherent (agent-scope)
 // stores, the workgroup waits for their completion, and one thread then
@@ -1456,7 +1387,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
                                                    const T *__restrict__ dinv, cplx *x, int *flags, int epoch,
                                                    int *status, BatchStrides bs) {
   constexpr int NB = kNB;
-  static_assert(SB == 2, "super-blocks are pairs of blocks");
+  constexpr int NI = SB * (SB - 1) / 2;
   A += blockIdx.z * bs.a;
   dinv += blockIdx.z * bs.inv;
   x += blockIdx.z * bs.vec;
@@ -1468,9 +1399,11 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
   const int cq = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform
   constexpr int CH = 8;  // far blocks taken per round
   __shared__ cplx xs[CH * NB];
+  __shared__ cplx part[4][NB];
   __shared__ cplx part2[SB][4][NB];
   __shared__ int nready_s;
   __shared__ cplx rhs[SB][NB];
+  __shared__ cplx xsb[SB][NB];
   // right-hand sides of the super-block (only this workgroup writes them)
   cplx b0[SB];
 #pragma unroll
@@ -1482,24 +1415,25 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
 #pragma unroll
     for(int cc = 0; cc < 16; ++cc) dst[cc] = col[(size_t)cc * ld + (unsigned)r];
   };
-  // diagonal-block inverse of block k, the off-diagonal block of the
-  // super-block's inverse, and the tile L[k, kc] (column-major in A)
+  // diagonal-block inverse of block k, and the tile L[k, kc] (column-major in A)
   auto inv_base = [&](int k) { return dinv + (size_t)k * 2 * NB * NB + (LOWER ? 0 : NB * NB); };
-  auto x_base = [&]() {
-    return reinterpret_cast<const cplx *>(dinv + 2 * (size_t)NB * Np) + (size_t)(2 * K + (LOWER ? 0 : 1)) * NB * NB;
-  };
   auto a_base = [&](int k, int kc) { return A + (size_t)kc * NB * lda + (size_t)k * NB; };
-  // near-phase tiles: mv[s] = inverse of block kb0+s, xt = X (two blocks)
-  T mv[SB][16];
-  cplx xt[16];
+  // near-phase tiles: mv[s] = inverse of block kb0+s; lv[idx(s, s')] = L[kb0+s, kb0+s']
+  T mv[PRE ? SB : 1][16], lv[PRE ? (NI > 0 ? NI : 1) : 1][16];
   auto preload = [&]() {
-    tile(inv_base(kb0), NB, mv[0]);
-    if(nb == 2) {
-      tile(inv_base(kb0 + 1), NB, mv[1]);
-      const cplx *col = x_base() + (size_t)(cq * 16) * NB;
 #pragma unroll
-      for(int cc = 0; cc < 16; ++cc) xt[cc] = col[(size_t)cc * NB + (unsigned)r];
-    }
+    for(int s = 0; s < SB; ++s)
+      if(s < nb) tile(inv_base(kb0 + s), NB, mv[PRE ? s : 0]);
+    int q = 0;
+#pragma unroll
+    for(int s = 0; s < SB; ++s)
+#pragma unroll
+      for(int s2 = 0; s2 < SB; ++s2) {
+        if(LOWER ? (s2 < s) : (s2 > s)) {
+          if(s < nb && s2 < nb) tile(a_base(kb0 + s, kb0 + s2), lda, lv[PRE ? q : 0]);
+          ++q;
+        }
+      }
   };
   if(PRE == 1) preload();
   const int nfar = LOWER ? kb0 : nblk - (kb0 + nb);
@@ -1575,43 +1509,79 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
       if(s < nb) rhs[s][t] = b0[s] - (part2[s][0][t] + part2[s][1][t] + part2[s][2][t] + part2[s][3][t]);
   }
   __syncthreads();
-  // ---- near phase: [x0; x1] = (super-block inverse) [r0; r1]
-  //   lower  x0 = M0 r0,  x1 = X r0 + M1 r1;   upper  x0 = M0 r0 + X r1,  x1 = M1 r1
-  {
-    cplx y0 = mk(0.0), y1 = mk(0.0);
-    auto mac = [&](cplx &y, const auto &m, int s) {
+  // ---- near phase: the super-block's blocks in order
+#pragma unroll
+  for(int i = 0; i < SB; ++i) {
+    const int s = LOWER ? i : SB - 1 - i;  // compile-time index; blocks past nb are skipped
+    if(s < nb) {
+      const int k = kb0 + s;
+      // contributions of the blocks of this super-block solved before
+      cplx a = mk(0.0);
+      int q = 0;
+      bool any = false;
+#pragma unroll
+      for(int ss = 0; ss < SB; ++ss)
+#pragma unroll
+        for(int s2 = 0; s2 < SB; ++s2) {
+          if(LOWER ? (s2 < ss) : (s2 > ss)) {
+            if(ss == s && s2 < nb) {
+              any = true;
+              T lt[16];
+              if(PRE) {
+#pragma unroll
+                for(int cc = 0; cc < 16; ++cc) lt[cc] = lv[PRE ? q : 0][cc];
+              } else {
+                tile(a_base(k, kb0 + s2), lda, lt);
+              }
+#pragma unroll
+              for(int cc = 0; cc < 16; ++cc) {
+                const cplx xv = xsb[s2][cq * 16 + cc];
+                a.x += (double)lt[cc].x * xv.x - (double)lt[cc].y * xv.y;
+                a.y += (double)lt[cc].x * xv.y + (double)lt[cc].y * xv.x;
+              }
+            }
+            ++q;
+          }
+        }
+      if(any) {
+        part[cq][r] = a;
+        __syncthreads();
+        if(t < NB) xs[t] = rhs[s][t] - (part[0][t] + part[1][t] + part[2][t] + part[3][t]);
+      } else {
+        if(t < NB) xs[t] = rhs[s][t];
+      }
+      __syncthreads();
+      // x_k = (diagonal block)^{-1} r
+      T m[16];
+      if(PRE) {
+#pragma unroll
+        for(int cc = 0; cc < 16; ++cc) m[cc] = mv[PRE ? s : 0][cc];
+      } else {
+        tile(inv_base(k), NB, m);
+      }
+      cplx y = mk(0.0);
 #pragma unroll
       for(int cc = 0; cc < 16; ++cc) {
-        const cplx v = rhs[s][cq * 16 + cc];
-        y.x += (double)m[cc].x * v.x - (double)m[cc].y * v.y;
-        y.y += (double)m[cc].x * v.y + (double)m[cc].y * v.x;
+        const cplx sv = xs[cq * 16 + cc];
+        y.x += (double)m[cc].x * sv.x - (double)m[cc].y * sv.y;
+        y.y += (double)m[cc].x * sv.y + (double)m[cc].y * sv.x;
       }
-    };
-    if(!PRE) preload();
-    mac(y0, mv[0], 0);
-    if(nb == 2) {
-      mac(y1, mv[1], 1);
-      if(LOWER)
-        mac(y1, xt, 0);
-      else
-        mac(y0, xt, 1);
+      part[cq][r] = y;
+      __syncthreads();
+      if(t < NB) {
+        const cplx xk = part[0][t] + part[1][t] + part[2][t] + part[3][t];
+        xsb[s][t] = xk;
+        unsigned long long *dst = reinterpret_cast<unsigned long long *>(x + (size_t)k * NB + t);
+        __hip_atomic_store(dst, (unsigned long long)__double_as_longlong(xk.x), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(dst + 1, (unsigned long long)__double_as_longlong(xk.y), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+      // the stores above have completed (device-coherent) before the flag
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __syncthreads();
+      if(t == 0) __hip_atomic_store(&flags[k], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    part2[0][cq][r] = y0;
-    part2[1][cq][r] = y1;
-    __syncthreads();
-    if(t < NB * nb) {
-      const int s = t / NB, i = t % NB;
-      const cplx xk = part2[s][0][i] + part2[s][1][i] + part2[s][2][i] + part2[s][3][i];
-      unsigned long long *dst = reinterpret_cast<unsigned long long *>(x + (size_t)(kb0 + s) * NB + i);
-      __hip_atomic_store(dst, (unsigned long long)__double_as_longlong(xk.x), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(dst + 1, (unsigned long long)__double_as_longlong(xk.y), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
-    // the stores above have completed (device-coherent) before the flags
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __syncthreads();
-    if(t < nb) __hip_atomic_store(&flags[kb0 + t], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

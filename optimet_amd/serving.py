@@ -41,7 +41,7 @@ def footprint(cluster, batch, kind):
     Np = -(-cluster.N // 64) * 64
     per_matrix = Np * Np * (24 if kind == "lu32ir" else 32)
     mats = 2 if cluster.sh else 1
-    vectors = Np * (16 * 16 + 2 * 16 * 3 * 64 + 16 * 41 + 16 * 32)
+    vectors = Np * (16 * 16 + 2 * 16 * 2 * 64 + 16 * 41 + 16 * 32)
     return batch * (mats * per_matrix + vectors)
 
 
@@ -87,6 +87,7 @@ class Server:
                 budget = int(0.6 * free)
         self.budget = int(budget if budget is not None else 1 << 40)
         self._lock = threading.Condition()
+        self._on_done = None
         self._held = 0
         self._free_slots = list(range(slots))
 
@@ -119,16 +120,20 @@ class Server:
         except Exception as e:  # reported per request; the server keeps going
             res.error = repr(e)
         res.finished = time.perf_counter() - self._t0
+        if self._on_done is not None:
+            self._on_done(res)
         with self._lock:
             self._held -= res.bytes
             self._free_slots.append(slot)
             self._lock.notify_all()
 
-    def run(self, requests, realtime=True, deadline=None):
+    def run(self, requests, realtime=True, deadline=None, on_done=None):
         """Serves the requests (arrival times in seconds from the start;
         realtime=False admits each one as soon as it fits; requests not
-        admitted by `deadline` seconds are reported as not served).  Returns
-        the results in request order."""
+        admitted by `deadline` seconds are reported as not served;
+        on_done(result) is called from the worker as each request ends).
+        Returns the results in request order."""
+        self._on_done = on_done
         reqs = sorted(requests, key=lambda r: r.arrival)
         results = {r.id: Result(r) for r in reqs}
         pending = collections.deque(reqs)
