@@ -895,7 +895,11 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
     gm.prec(rhs, x);
     std::vector<double> res;
     std::vector<int> its;
-    gm.solve(rhs, x, c->tol, c->maxit, c->restart, res, its);
+    // the batch runs in lockstep until its slowest system stops: a system
+    // still short of the target after four restart cycles is left to the
+    // one-at-a-time path (partial pivoting) instead of holding the batch
+    const int bmax = nb > 1 ? std::min(c->maxit, 4 * c->restart) : c->maxit;
+    gm.solve(rhs, x, c->tol, bmax, c->restart, res, its);
     status_scan(bad);
     for(int z = 0; z < nb; ++z) {
       if(!(res[z] <= 1e3 * c->tol)) bad[z] = 1;
