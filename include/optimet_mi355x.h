@@ -175,8 +175,9 @@ int omx_read_vector(omx_context *ctx, int which, double *out);
  * sphere's relative permittivity at each wavelength (at 2 omega for eps_r_sh);
  * NULL keeps the geometry's values.  Up to `batch` systems (0: automatic, at
  * most 64 and half the free device memory) are assembled, factorised (block
- * pivoting) and refined together; a system with a zero pivot or a refinement
- * that does not converge is redone alone with partial pivoting.  Outputs (any
+ * pivoting) and refined together; a system with a zero pivot, or still short
+ * of the tolerance after four restart cycles of the lockstep refinement, is
+ * redone alone with partial pivoting.  Outputs (any
  * may be NULL): x_sca, x_int, x_sca_sh, x_int_sh nl*N complex each, xs nl*4
  * cross sections as omx_cross_sections, iterations nl*2 (omega, 2 omega). */
 int omx_solve_sweep(omx_context *ctx, int nl, const double *lambdas, const double *eps_r, const double *eps_r_sh,
