@@ -318,11 +318,12 @@ struct Gmres {
       double beta = nrm(r);
       *res = beta / bn;
       if(*res <= tol || it >= maxit || tiny_step) break;
-      // stagnation at the FP64 floor: two restart cycles without halving, once
-      // the residual is down to 1e-10 (further up a slow phase of the
-      // restarted iteration is waited out, up to maxit)
+      // stagnation at the FP64 floor: a restart cycle without halving once the
+      // residual is within the acceptance band (<= 1e3 tol), two once it is
+      // down to 1e-10 (further up a slow phase of the restarted iteration is
+      // waited out, up to maxit)
       if(prev > 0.0 && beta > 0.5 * prev && *res < 1e-10) {
-        if(++stall >= 2) break;
+        if(++stall >= 2 || *res <= 1e3 * tol) break;
       } else {
         stall = 0;
       }
@@ -617,7 +618,9 @@ struct GmresB {
           continue;
         }
         if(prev[z] > 0.0 && beta[z] > 0.5 * prev[z]) {
-          if(++stall[z] >= 2) {
+          // within the acceptance band one cycle without halving is the
+          // FP64 floor; further up, two
+          if(++stall[z] >= 2 || res[z] <= 1e3 * tol) {
             active[z] = 0;
             continue;
           }
