@@ -109,7 +109,7 @@ struct omx_context {
   DBuf d_status2, d_map2;
   std::vector<hipEvent_t> pev2;
   omx::LuProf prof2{};
-  DBuf d_basis, d_w1, d_w2, d_r, d_part, d_h, d_nrm, d_gpart;
+  DBuf d_basis, d_basisZ, d_w1, d_w2, d_r, d_part, d_h, d_nrm, d_gpart;
   omx::LuWork lu{}, lu_sh{};
   bool profile = false;
   std::vector<hipEvent_t> pev;
@@ -121,7 +121,8 @@ struct omx_context {
   // wavelength sweeps: per-system buffers of one batch ([system][...])
   struct Sweep {
     DBuf S64, S32, V64, V32, F64, F64sh, eps, eps_sh, ov, incloc, Q, w, x, xint, xsh, xintsh, K, K1, cross;
-    DBuf perm, perm2, flags, flags2, dinv, dinv2, map, status, basis, r, w1, w2, gpart, part, hd, nrm, sc, act;
+    DBuf perm, perm2, flags, flags2, dinv, dinv2, map, status, basis, basisZ, r, w1, w2, gpart, part, hd, nrm, sc,
+        act;
   } sw;
   omx_stats stats{};
   long long mv_calls = 0;  // FP64 matrix-vector products since the last omx_set_profile
@@ -255,6 +256,7 @@ void alloc_matrices(omx_context *c) {
   if(f64) c->d_F64.ensure(sizeof(cplx) * n2, T); else c->d_F64.release(T);
   if(f64 && c->sh) c->d_F64sh.ensure(sizeof(cplx) * n2, T); else c->d_F64sh.release(T);
   c->d_basis.ensure(sizeof(cplx) * (size_t)c->Np * (c->restart + 1), T);
+  c->d_basisZ.ensure(sizeof(cplx) * (size_t)c->Np * (c->restart + 1), T);
   alloc_lu(c, c->lu, c->d_piv, c->d_perm, c->d_flags, c->d_dinv, false);
   if(c->sh) alloc_lu(c, c->lu_sh, c->d_piv2, c->d_perm2, c->d_flags2, c->d_dinv2, true);
 }
@@ -298,7 +300,9 @@ struct Gmres {
   // Solves A x = b; x holds the initial guess.  Returns iterations; *res the
   // final true relative residual.
   int solve(const cplx *b, cplx *x, double tol, int maxit, int m, double *res) {
-    cplx *V = c->d_basis.as<cplx>();
+    // flexible form: the preconditioned vectors M^{-1} v_j are kept (Zb), so
+    // the end of a cycle needs no further preconditioner application
+    cplx *V = c->d_basis.as<cplx>(), *Zb = c->d_basisZ.as<cplx>();
     cplx *r = c->d_r.as<cplx>(), *w = c->d_w1.as<cplx>(), *z = c->d_w2.as<cplx>();
     cplx *hd = c->d_h.as<cplx>(), *part = c->d_part.as<cplx>();
     const size_t ld = (size_t)c->Np;
@@ -335,8 +339,8 @@ struct Gmres {
       for(; j < m && it < maxit; ++j, ++it) {
         const cplx *vj = V + (size_t)j * ld;
         if(prec) {
-          prec(vj, z);
-          matvec(z, w);
+          prec(vj, Zb + (size_t)j * ld);
+          matvec(Zb + (size_t)j * ld, w);
         } else {
           matvec(vj, w);
         }
@@ -387,12 +391,8 @@ struct Gmres {
       std::vector<cplx> yh(j);
       for(int i = 0; i < j; ++i) yh[i] = omx::mk(y[i].real(), y[i].imag());
       HIPCHK(hipMemcpyAsync(hd, yh.data(), sizeof(cplx) * j, hipMemcpyHostToDevice, s));
-      omx::gemv_multi(V, ld, j, hd, z, N, s);
+      omx::gemv_multi(prec ? Zb : V, ld, j, hd, z, N, s);
       const cplx *dx = z;
-      if(prec) {
-        prec(z, w);
-        dx = w;
-      }
       omx::vaxpby(x, omx::mk(1.0), dx, omx::mk(1.0), N, s);
       // a correction below the FP64 resolution of x ends the refinement
       if(nrm(dx) <= 1e-16 * nrm(x)) tiny_step = true;
@@ -579,9 +579,9 @@ struct GmresB {
   void solve(const cplx *b, cplx *x, double tol, int maxit, int m, std::vector<double> &res, std::vector<int> &its) {
     // basis vector k of system z at V + (k nb + z) Np: the systems' k-th
     // vectors are contiguous, as the batched preconditioner expects
-    cplx *V = w.basis.as<cplx>();
+    cplx *V = w.basis.as<cplx>(), *Zb = w.basisZ.as<cplx>();  // Zb: M^{-1} V (flexible form)
     const size_t sV = (size_t)Np, ld = (size_t)Np * nb;
-    cplx *r = w.r.as<cplx>(), *wv = w.w1.as<cplx>(), *zv = w.w2.as<cplx>();
+    cplx *r = w.r.as<cplx>(), *wv = w.w1.as<cplx>();
     cplx *hd = w.hd.as<cplx>(), *part = w.part.as<cplx>();
     const int sh = m + 2;
     int *act_d = w.act.as<int>();
@@ -644,8 +644,9 @@ struct GmresB {
       }
       for(int j = 0; j < m && it < maxit; ++j, ++it) {
         const cplx *vj = V + (size_t)j * ld;
-        prec(vj, zv);
-        matvec(zv, wv);
+        cplx *zj = Zb + (size_t)j * ld;
+        prec(vj, zj);
+        matvec(zj, wv);
         std::vector<cplx> h1((size_t)nb * sh), h2((size_t)nb * sh);
         for(int pass = 0; pass < 2; ++pass) {
           omx::dotc_multi_b(V, ld, sV, j + 1, wv, sv(), Np, nb, part, hd, sh, c->st);
@@ -697,7 +698,7 @@ struct GmresB {
           break;
         }
       }
-      // y_z = H_z^{-1} g_z, x_z += M^{-1} V_z y_z
+      // y_z = H_z^{-1} g_z, x_z += Z_z y_z (= M^{-1} V_z y_z)
       int jmax = 0;
       std::vector<cplx> yh((size_t)nb * sh, zero);
       for(int z = 0; z < nb; ++z) {
@@ -714,8 +715,7 @@ struct GmresB {
       }
       if(jmax == 0) break;
       HIPCHK(hipMemcpyAsync(hd, yh.data(), sizeof(cplx) * nb * sh, hipMemcpyHostToDevice, c->st));
-      omx::gemv_multi_b(V, ld, sV, jmax, hd, sh, zv, sv(), Np, nb, c->st);
-      prec(zv, wv);
+      omx::gemv_multi_b(Zb, ld, sV, jmax, hd, sh, wv, sv(), Np, nb, c->st);
       std::vector<int> act_h(active);
       HIPCHK(hipMemcpyAsync(act_d, act_h.data(), sizeof(int) * nb, hipMemcpyHostToDevice, c->st));
       scale(x, sv(), std::vector<cplx>(nb, one), wv, sv(), std::vector<cplx>(nb, one), act_d);
@@ -758,6 +758,7 @@ void sweep_alloc(omx_context *c, int nb, bool f64, bool sh) {
   w.map.ensure(sizeof(int) * 3 * (Np + 2 * omx::kNB) * nb, T);
   w.status.ensure(sizeof(int) * 4 * nb, T);
   w.basis.ensure(sizeof(cplx) * Np * (m + 1) * nb, T);
+  w.basisZ.ensure(sizeof(cplx) * Np * (m + 1) * nb, T);
   w.gpart.ensure(sizeof(cplx) * Np * omx::kGemvSplitMax * nb, T);
   w.part.ensure(sizeof(cplx) * 128 * (m + 2) * nb, T);
   w.hd.ensure(sizeof(cplx) * (m + 2) * nb, T);
@@ -1018,14 +1019,14 @@ void omx_destroy(omx_context *c) {
                   &c->d_sph, &c->d_ov, &c->d_inc, &c->d_incloc, &c->d_Q, &c->d_w, &c->d_coef, &c->d_S64,
                   &c->d_S32, &c->d_V64, &c->d_V32, &c->d_x, &c->d_xsh, &c->d_xint, &c->d_xintsh, &c->d_K,
                   &c->d_K1, &c->d_cross, &c->d_cg, &c->d_piv, &c->d_perm, &c->d_cand, &c->d_cnt, &c->d_map,
-                  &c->d_wlu, &c->d_status, &c->d_flags, &c->d_basis, &c->d_w1, &c->d_w2, &c->d_r,
+                  &c->d_wlu, &c->d_status, &c->d_flags, &c->d_basis, &c->d_basisZ, &c->d_w1, &c->d_w2, &c->d_r,
                   &c->d_part, &c->d_h, &c->d_nrm, &c->d_gpart, &c->d_piv2, &c->d_perm2, &c->d_flags2,
                   &c->d_dinv, &c->d_dinv2, &c->d_F64, &c->d_F64sh};
   for(DBuf *b : bufs) b->release(&c->total);
   auto &w = c->sw;
   DBuf *sbufs[] = {&w.S64, &w.S32, &w.V64, &w.V32, &w.F64, &w.F64sh, &w.eps, &w.eps_sh, &w.ov, &w.incloc, &w.Q,
                    &w.w, &w.x, &w.xint, &w.xsh, &w.xintsh, &w.K, &w.K1, &w.cross, &w.perm, &w.perm2, &w.flags,
-                   &w.flags2, &w.dinv, &w.dinv2, &w.map, &w.status, &w.basis, &w.r, &w.w1, &w.w2, &w.gpart,
+                   &w.flags2, &w.dinv, &w.dinv2, &w.map, &w.status, &w.basis, &w.basisZ, &w.r, &w.w1, &w.w2, &w.gpart,
                    &w.part, &w.hd, &w.nrm, &w.sc, &w.act};
   for(DBuf *b : sbufs) b->release(&c->total);
   for(auto &e : c->ev)
@@ -1585,7 +1586,8 @@ int omx_trim(omx_context *c) {
     HIPCHK(hipStreamSynchronize(c->st));
     auto &w = c->sw;
     DBuf *big[] = {&c->d_S64, &c->d_S32, &c->d_V64, &c->d_V32, &c->d_F64, &c->d_F64sh, &w.S64, &w.S32, &w.V64,
-                   &w.V32, &w.F64, &w.F64sh, &w.basis, &w.gpart, &c->d_basis, &c->d_gpart};
+                   &w.V32, &w.F64, &w.F64sh, &w.basis, &w.basisZ, &w.gpart, &c->d_basis, &c->d_basisZ,
+                   &c->d_gpart};
     for(DBuf *b : big) b->release(&c->total);
     c->updated = c->solved = false;
     c->factored = c->factored_sh = false;

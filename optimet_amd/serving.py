@@ -37,11 +37,11 @@ def footprint(cluster, batch, kind):
     """Device bytes of a batch of `batch` systems of this cluster: the FP64
     matrix (16 N^2) and, per solver kind, its FP32 copy (8 N^2) or FP64
     factors (16 N^2), for S and (with the second harmonic) V, plus the
-    per-system vectors and panel inverses."""
+    per-system vectors, panel inverses and the two Krylov bases (restart 40)."""
     Np = -(-cluster.N // 64) * 64
     per_matrix = Np * Np * (24 if kind == "lu32ir" else 32)
     mats = 2 if cluster.sh else 1
-    vectors = Np * (16 * 16 + 2 * 16 * 2 * 64 + 16 * 41 + 16 * 32)
+    vectors = Np * (16 * 16 + 2 * 16 * 2 * 64 + 2 * 16 * 41 + 16 * 32)
     return batch * (mats * per_matrix + vectors)
 
 
