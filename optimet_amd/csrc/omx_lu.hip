@@ -1155,6 +1155,12 @@ template <typename T>
 void bulk_update(T *A, size_t lda, int row0, int col0, int k0, int M, int N, hipStream_t st, int nb, size_t sA,
                  bool split) {
   if(M <= 0 || N <= 0) return;
+  if(N <= 2 * kNB) {
+    // the next pair's columns (on the panel chain's critical path): small
+    // tiles over many workgroups finish sooner than a few large split tiles
+    trailing_update<T, 2 * kNB, ThinTile>(A, lda, row0, col0, k0, M, N, st, nb, sA);
+    return;
+  }
   if constexpr(std::is_same<T, float2>::value) {
     if(split) {
       using U = SplitTile;
