@@ -170,6 +170,29 @@ def test_lu32_gemm_modes_cluster(ctx, ora):
         ctx.set_lu32_gemm("bf16x3")
 
 
+def test_mixed_refinement_ill_conditioned(ctx):
+    """cond(A) = 1e6: the FP32 factors are a weak preconditioner
+    (||I - M^{-1} A|| about cond * 6e-8, near 0.1), and the flexible refinement
+    still reaches the FP64 answer (to cond * 1e-16 relative), as the FP64 LU
+    does.  b = A x for a random x, so that ||A|| ||x|| / ||b|| is of order
+    one: the refinement's relative residual ||b - A x|| / ||b|| then has an
+    FP64 floor near 1e-16 (for a b with ||x|| ~ cond ||b|| the floor is
+    cond * 1e-16, above the acceptance band of tol = 1e-16).  Beyond
+    cond * 6e-8 ~ 1 the FP32 factors stop being a preconditioner."""
+    rng = np.random.default_rng(77)
+    N = 512
+    U, _ = np.linalg.qr(rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)))
+    V, _ = np.linalg.qr(rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)))
+    A = (U * np.logspace(0, -6, N)) @ V.conj().T
+    xt = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    b = A @ xt
+    xr = np.linalg.solve(A, b)
+    for kind in ("lu64", "lu32ir"):
+        x = ctx.dense_solve(A, b, kind)
+        assert rel(x, xr) < 1e-9, kind
+    assert ctx.stats()["iterations"] > 5  # the mixed refinement had real work to do
+
+
 def test_dense_solve_needs_pivoting(ctx):
     """Zero leading diagonal: only a pivoting LU survives."""
     rng = np.random.default_rng(3)
