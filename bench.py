@@ -49,7 +49,8 @@ def traffic_per_launch():
                 if "=" in line and not line.startswith("#"):
                     k, v = line.split("=", 1)
                     vals[k.strip()] = float(v.split()[0])
-        return vals["fetch_bytes_per_launch"] + vals["write_bytes_per_launch"]
+        return {"bytes_per_split_launch": vals["fetch_bytes_per_launch"] + vals["write_bytes_per_launch"],
+                "algorithmic_bytes_per_split_launch": vals["algorithmic_bytes_per_launch"]}
     except (OSError, KeyError, ValueError):
         return None
 
@@ -144,7 +145,7 @@ def sweep_main(args):
         "metric": "cluster solves/sec + p99 latency, mixed-precision vs FP64 LU at rel err <=1e-9",
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * mixed["total"] / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "c64 LU (MFMA) + c128 GMRES-IR", "data": "synthetic (seeded Drude cluster)",
+        "vs_baseline": None, "dtype": "c64 LU (FP32-accurate bf16x3 MFMA products) + c128 GMRES-IR", "data": "synthetic (seeded Drude cluster)",
         "config": {"workload": c.meta["name"], "P": c.nobj, "nmax": c.nmax, "N": c.N, "wavelengths": nl,
                    "frequencies": "omega+2omega", "solver": "lu32ir", "batch": ms["sweep_batch"]},
         "step": "one sweep of all wavelengths (a solve = one wavelength at omega and 2 omega)",
@@ -210,7 +211,7 @@ def config4_main(args):
         "metric": "cluster solves/sec + p99 latency, mixed-precision vs FP64 LU at rel err <=1e-9",
         "value": 1.0 / (m["update_s"] + m["solve_s"]), "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": 1e3 * (m["update_s"] + m["solve_s"]), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "c64 LU (MFMA) + c128 GMRES-IR", "data": "synthetic (seeded Drude cluster)",
+        "vs_baseline": None, "dtype": "c64 LU (FP32-accurate bf16x3 MFMA products) + c128 GMRES-IR", "data": "synthetic (seeded Drude cluster)",
         "config": {"workload": c.meta["name"], "P": c.nobj, "nmax": c.nmax, "N": c.N, "frequencies": "omega",
                    "solver": "lu32ir"},
         "roofline": m.get("roofline"),
@@ -257,7 +258,7 @@ def config5_main(args):
         "metric": "cluster solves/sec + p99 latency, mixed-precision vs FP64 LU at rel err <=1e-9",
         "value": m["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": 1e3 * m["span_s"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c64 LU (MFMA) + c128 GMRES-IR", "data": "synthetic (seeded request stream)",
+        "dtype": "c64 LU (FP32-accurate bf16x3 MFMA products) + c128 GMRES-IR", "data": "synthetic (seeded request stream)",
         "config": {"workload": "config5-serving-stream", "requests": args.requests, "slots": args.slots,
                    "deadline_s": args.deadline, "rate_hz": 50.0, "solver": "lu32ir"},
         "step": "the served sample; a solve = one wavelength of a request at omega and 2 omega",
@@ -340,7 +341,12 @@ def main():
     roof = roofline(ms, "lu32ir", SPLIT_TFLOPS)
     if roof is not None:
         roof["peak_basis"] = "bf16 dense 2500 TFLOP/s / 6 bf16 products per FP32-accurate real product"
-        roof["traffic"] = traffic_per_launch()
+        tr = traffic_per_launch()
+        roof["traffic"] = tr["bytes_per_split_launch"] if tr else None
+        if tr:
+            roof["traffic_note"] = ("HBM bytes per gemm_split launch (the wide part of a pair's bulk update; "
+                                    f"algorithmic {tr['algorithmic_bytes_per_split_launch']:.3g} B), rocprofv3 "
+                                    "FETCH_SIZE + WRITE_SIZE, copy-calibrated")
     line = {
         "metric": "cluster solves/sec + p99 latency, mixed-precision vs FP64 LU at rel err <=1e-9",
         "value": value,
@@ -352,7 +358,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 LU (MFMA) + c128 GMRES-IR",
+        "dtype": "c64 LU (FP32-accurate bf16x3 MFMA products) + c128 GMRES-IR",
         "data": "synthetic (seeded Drude cluster)",
         "config": {"workload": c.meta["name"], "P": c.nobj, "nmax": c.nmax, "N": c.N,
                    "wavelength_nm": c.wavelength * 1e9, "frequencies": "omega+2omega", "solver": "lu32ir"},
