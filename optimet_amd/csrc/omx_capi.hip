@@ -110,6 +110,9 @@ struct omx_context {
   std::vector<hipEvent_t> pev2;
   omx::LuProf prof2{};
   DBuf d_basis, d_basisZ, d_w1, d_w2, d_r, d_part, d_h, d_nrm, d_gpart;
+  // pinned host staging of one Arnoldi step's projections and norm
+  cplx *h_pin = nullptr;
+  int h_pin_n = 0;
   omx::LuWork lu{}, lu_sh{};
   bool profile = false;
   std::vector<hipEvent_t> pev;
@@ -234,7 +237,7 @@ void alloc_all(omx_context *c) {
   c->d_w2.ensure(sizeof(cplx) * Np, T);
   c->d_gpart.ensure(sizeof(cplx) * Np * omx::kGemvSplitMax, T);
   c->d_part.ensure(sizeof(cplx) * 128 * (c->restart + 2), T);
-  c->d_h.ensure(sizeof(cplx) * (c->restart + 2), T);
+  c->d_h.ensure(sizeof(cplx) * 2 * (c->restart + 2), T);
   c->d_nrm.ensure(sizeof(double) * 256, T);
 }
 
@@ -312,6 +315,14 @@ struct Gmres {
       *res = 0.0;
       return 0;
     }
+    if(c->h_pin_n < 2 * (m + 2) + 1) {
+      if(c->h_pin) (void)hipHostFree(c->h_pin);
+      c->h_pin = nullptr;
+      c->h_pin_n = 0;
+      HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&c->h_pin), sizeof(cplx) * (2 * (m + 2) + 1)));
+      c->h_pin_n = 2 * (m + 2) + 1;
+    }
+    cplx *hpin = c->h_pin;
     int it = 0, stall = 0;
     double prev = 0.0;
     bool tiny_step = false;
@@ -344,14 +355,19 @@ struct Gmres {
         } else {
           matvec(vj, w);
         }
-        std::vector<cplx> h1(j + 1), h2(j + 1);
+        // two Gram-Schmidt passes and the norm, copied to pinned memory with
+        // one wait for all three
         for(int pass = 0; pass < 2; ++pass) {
-          omx::dotc_multi(V, ld, j + 1, w, N, part, hd, s);
-          omx::axpy_multi(V, ld, j + 1, hd, w, N, s);
-          HIPCHK(hipMemcpyAsync(pass ? h2.data() : h1.data(), hd, sizeof(cplx) * (j + 1), hipMemcpyDeviceToHost,
-                                s));
+          cplx *hp = hd + (size_t)pass * (m + 2);
+          omx::dotc_multi(V, ld, j + 1, w, N, part, hp, s);
+          omx::axpy_multi(V, ld, j + 1, hp, w, N, s);
+          HIPCHK(hipMemcpyAsync(hpin + (size_t)pass * (m + 2), hp, sizeof(cplx) * (j + 1), hipMemcpyDeviceToHost, s));
         }
-        double hn = nrm(w);
+        omx::nrm2sq(w, N, c->d_nrm.as<double>() + 1, c->d_nrm.as<double>(), s);
+        HIPCHK(hipMemcpyAsync(hpin + 2 * (m + 2), c->d_nrm.p, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const cplx *h1 = hpin, *h2 = hpin + (m + 2);
+        double hn = std::sqrt(reinterpret_cast<const double *>(hpin + 2 * (m + 2))[0]);
         for(int i = 0; i <= j; ++i)
           H[(size_t)j * (m + 1) + i] = std::complex<double>(h1[i].x + h2[i].x, h1[i].y + h2[i].y);
         H[(size_t)j * (m + 1) + j + 1] = hn;
@@ -1045,6 +1061,7 @@ void omx_destroy(omx_context *c) {
   if(c->st2) (void)hipStreamDestroy(c->st2);
   if(c->aux) (void)hipStreamDestroy(c->aux);
   if(c->st) (void)hipStreamDestroy(c->st);
+  if(c->h_pin) (void)hipHostFree(c->h_pin);
   delete c;
 }
 
@@ -1420,7 +1437,7 @@ int omx_dense_solve(omx_context *c, int N, const double *A, const double *b, dou
     ctx.d_w2.ensure(sizeof(cplx) * Np, &ctx.total);
     ctx.d_gpart.ensure(sizeof(cplx) * Np * omx::kGemvSplitMax, &ctx.total);
     ctx.d_part.ensure(sizeof(cplx) * 128 * (ctx.restart + 2), &ctx.total);
-    ctx.d_h.ensure(sizeof(cplx) * (ctx.restart + 2), &ctx.total);
+    ctx.d_h.ensure(sizeof(cplx) * 2 * (ctx.restart + 2), &ctx.total);
     ctx.d_nrm.ensure(sizeof(double) * 256, &ctx.total);
     alloc_matrices(&ctx);
     HIPCHK(hipMemsetAsync(ctx.d_Q.p, 0, sizeof(cplx) * Np, ctx.st));
