@@ -543,9 +543,26 @@ void solve_system(omx_context *c, bool shf, const cplx *rhs, cplx *x, int *iters
       *iters = gm.solve(rhs, x, c->tol, c->maxit, c->restart, res);
       check_status(c, shf);
       // the target may sit below the FP64 residual floor of an ill-conditioned
-      // system; a stagnated refinement within 1000x of it is accepted
-      if(!(*res <= 1e3 * c->tol))
-        throw OmxError(OMX_ERR_NOCONV, "refinement stopped at relative residual " + std::to_string(*res));
+      // system; a stagnated refinement within 1000x of it is accepted, and so
+      // is one whose normwise backward error ||b - A x|| / (||A||_F ||x|| + ||b||)
+      // is within 1000x of the FP64 unit roundoff (the relative residual's
+      // floor is u ||A|| ||x|| / ||b||, far above u when x is large)
+      if(!(*res <= 1e3 * c->tol)) {
+        const double bn = gm.nrm(rhs), xn = gm.nrm(x);
+        double a2 = 0.0;
+        const size_t n2 = (size_t)Np * lda, chunk = (size_t)1 << 28;
+        for(size_t off = 0; off < n2; off += chunk) {
+          omx::nrm2sq(A64 + off, (int)std::min(chunk, n2 - off), c->d_nrm.as<double>() + 1, c->d_nrm.as<double>(), st);
+          double v = 0.0;
+          HIPCHK(hipMemcpyAsync(&v, c->d_nrm.p, sizeof(double), hipMemcpyDeviceToHost, st));
+          HIPCHK(hipStreamSynchronize(st));
+          a2 += v;
+        }
+        const double eta = (*res * bn) / (std::sqrt(a2) * xn + bn);
+        if(!(eta <= 1e3 * std::max(c->tol, 1.1e-16)))
+          throw OmxError(OMX_ERR_NOCONV, "refinement stopped at relative residual " + std::to_string(*res) +
+                                             ", backward error " + std::to_string(eta));
+      }
       return;
     } catch(OmxError &e) {
       if(last || (e.code != OMX_ERR_SINGULAR && e.code != OMX_ERR_NOCONV)) throw;

@@ -193,6 +193,24 @@ def test_mixed_refinement_ill_conditioned(ctx):
     assert ctx.stats()["iterations"] > 5  # the mixed refinement had real work to do
 
 
+def test_refinement_accepts_backward_stable_large_solution(ctx):
+    """cond(A) = 1e6 and a random b, so ||x|| ~ 1e6 ||b||: the relative
+    residual cannot go below about u ||A|| ||x|| / ||b|| ~ 1e-10, far above the
+    acceptance band of tol = 1e-16, but the normwise backward error reaches the
+    FP64 unit roundoff and the solvers return x (to cond * 1e-15 relative)
+    instead of reporting no convergence."""
+    rng = np.random.default_rng(78)
+    N = 512
+    U, _ = np.linalg.qr(rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)))
+    V, _ = np.linalg.qr(rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)))
+    A = (U * np.logspace(0, -6, N)) @ V.conj().T
+    b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    xr = np.linalg.solve(A, b)
+    for kind in ("lu64", "lu32ir"):
+        x = ctx.dense_solve(A, b, kind)
+        assert rel(x, xr) < 1e-8, kind
+
+
 def test_dense_solve_needs_pivoting(ctx):
     """Zero leading diagonal: only a pivoting LU survives."""
     rng = np.random.default_rng(3)
