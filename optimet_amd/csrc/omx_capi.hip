@@ -60,6 +60,10 @@ struct DBuf {
   template <typename T> T *as() const { return static_cast<T *>(p); }
 };
 
+// Normwise backward error accepted from a refinement that stopped above its
+// residual band: 1000 units of FP64 roundoff.
+const double kBackwardErrorBound = 1e3 * 1.1e-16;
+
 const double kMu0 = 4.0 * omx::kPi * 1e-7;
 const double kEps0 = 1.0 / (kMu0 * omx::kC0 * omx::kC0);
 
@@ -546,7 +550,9 @@ void solve_system(omx_context *c, bool shf, const cplx *rhs, cplx *x, int *iters
       // system; a stagnated refinement within 1000x of it is accepted, and so
       // is one whose normwise backward error ||b - A x|| / (||A||_F ||x|| + ||b||)
       // is within 1000x of the FP64 unit roundoff (the relative residual's
-      // floor is u ||A|| ||x|| / ||b||, far above u when x is large)
+      // floor is u ||A|| ||x|| / ||b||, far above u when x is large).  That
+      // bound is a constant: it does not grow with tol, so a caller's loose
+      // tol never lets an FP32-quality answer through.
       if(!(*res <= 1e3 * c->tol)) {
         const double bn = gm.nrm(rhs), xn = gm.nrm(x);
         double a2 = 0.0;
@@ -559,7 +565,7 @@ void solve_system(omx_context *c, bool shf, const cplx *rhs, cplx *x, int *iters
           a2 += v;
         }
         const double eta = (*res * bn) / (std::sqrt(a2) * xn + bn);
-        if(!(eta <= 1e3 * std::max(c->tol, 1.1e-16)))
+        if(!(eta <= kBackwardErrorBound))
           throw OmxError(OMX_ERR_NOCONV, "refinement stopped at relative residual " + std::to_string(*res) +
                                              ", backward error " + std::to_string(eta));
       }
@@ -1438,9 +1444,24 @@ int omx_dense_solve(omx_context *c, int N, const double *A, const double *b, dou
     if(!c || N < 1 || !A || !b || !x || (kind != OMX_SOLVER_LU64 && kind != OMX_SOLVER_LU32_IR))
       throw OmxError(OMX_ERR_ARG, "invalid dense solve");
     HIPCHK(hipSetDevice(c->device));
-    // reuse the context machinery with a synthetic "geometry" of the right size
+    // reuse the context machinery with a synthetic "geometry" of the right
+    // size; the geometry's sizes, solver kind and SH flag come back on every
+    // exit (a failed solve must not leave the dense sizes behind: the next
+    // omx_update would size its buffers from them)
     omx_context &ctx = *c;
-    int saveP = ctx.P, saveN = ctx.N, saveNp = ctx.Np, saveKind = ctx.kind, saveSh = ctx.sh;
+    struct Restore {
+      omx_context &c;
+      int N, Np, kind, sh;
+      ~Restore() {
+        c.N = N;
+        c.Np = Np;
+        c.kind = kind;
+        c.sh = sh;
+        c.updated = false;
+        c.solved = false;
+        c.factored = c.factored_sh = false;
+      }
+    } restore{ctx, ctx.N, ctx.Np, ctx.kind, ctx.sh};
     ctx.N = N;
     ctx.Np = (N + omx::kNB - 1) / omx::kNB * omx::kNB;
     ctx.kind = kind;
@@ -1479,13 +1500,6 @@ int omx_dense_solve(omx_context *c, int N, const double *A, const double *b, dou
     HIPCHK(hipStreamSynchronize(ctx.st));
     ctx.stats.iterations = it;
     ctx.stats.residual = res;
-    ctx.P = saveP;
-    ctx.N = saveN;
-    ctx.Np = saveNp;
-    ctx.kind = saveKind;
-    ctx.sh = saveSh;
-    ctx.updated = false;
-    ctx.factored = false;
   });
 }
 
@@ -1551,6 +1565,15 @@ int omx_solve_sweep(omx_context *c, int nl, const double *lambdas, const double 
     const float t_batch = elapsed(c, 6, 7);
     // systems the batch could not solve: one at a time with partial pivoting
     if(!redo.empty()) {
+      // the batch's matrix-sized buffers are released first, so the
+      // one-at-a-time solves (which allocate a single system's matrices,
+      // copies and bases) never hold memory beside a whole batch: the
+      // context's peak stays the batch footprint a scheduler admitted
+      {
+        auto &w = c->sw;
+        DBuf *big[] = {&w.S64, &w.S32, &w.V64, &w.V32, &w.F64, &w.F64sh, &w.basis, &w.basisZ, &w.gpart};
+        for(DBuf *b : big) b->release(&c->total);
+      }
       const double lam0 = c->lambda;
       const std::vector<cplx> e0 = c->eps, es0 = c->eps_sh;
       const int piv0 = c->pivot;

@@ -1582,10 +1582,21 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
                            __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(dst + 1, (unsigned long long)__double_as_longlong(xk.y), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
+        // Publication order.  Wave 0 (threads 0..63) issues every x store of
+        // the block and, below, the flag store.  The x stores are agent-scope
+        // (sc1, written through to the device-coherent level), and a store's
+        // vmcnt slot is released only when that write is acknowledged there,
+        // so waiting for vmcnt(0) here makes x_k visible to every XCD before
+        // the flag can be issued.  A workgroup-scope release fence emits no
+        // such wait, and an agent-scope release would add an L2 writeback
+        // that these write-through stores do not need.  The memory clobber
+        // keeps the compiler from moving memory operations across the wait.
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       }
-      // the stores above have completed (device-coherent) before the flag
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __syncthreads();
+      // Consumers poll the flag with agent-scope loads and read x_k with
+      // agent-scope loads issued only after the poll has returned the epoch
+      // (the ballot, branch and barrier depend on its value), so they see x_k.
       if(t == 0) __hip_atomic_store(&flags[k], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }

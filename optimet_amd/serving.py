@@ -19,7 +19,10 @@ Inside a request the wavelengths are solved as batched sweeps
 (`Solver.sweep`, i.e. `omx_solve_sweep`). The T-matrices, the assembled S and V
 and their factorisations stay resident from the ω solve to the 2ω solve. A
 worker trims its matrix buffers after each request, so the footprint it
-holds is the one it was admitted with.
+holds is the one it was admitted with. A system that the batch cannot solve
+is redone alone with partial pivoting; the sweep releases the batch's matrix
+buffers before that, so the request then holds one system's footprint, which
+is at most the batch's.
 
 The scheduler is plain Python over the C ABI (ctypes releases the GIL during
 the calls); `solve_fn` can be replaced for CPU tests of the admission logic.

@@ -152,6 +152,13 @@ class Context:
         the mixed-precision factorisation's bulk updates (OMX_GEMM_*)."""
         _check(self.L.omx_set_lu32_gemm(self.h, GEMMS[mode]))
 
+    def set_solver(self, kind="lu64", tol=1e-16, maxit=300, restart=40):
+        """omx_set_solver: solver kind and the iterative parts' settings."""
+        _check(self.L.omx_set_solver(self.h, KINDS[kind], tol, maxit, restart))
+
+    def scattering_size(self):
+        return self.L.omx_scattering_size(self.h)
+
     def dense_solve(self, A, b, kind="lu64"):
         A = np.asfortranarray(A, dtype=np.complex128)
         N = A.shape[0]

@@ -85,6 +85,24 @@ def test_batch_shrinks_to_fit():
     assert b == 4 and nbytes <= s.budget
 
 
+def test_admitted_footprint_covers_the_one_at_a_time_fallback():
+    """A sweep that redoes a system alone (omx_solve_sweep's partial-pivoting
+    fallback) first releases the batch's matrix buffers, so the request peaks
+    at max(footprint(batch), footprint(1)): the footprint plan() admits it
+    with covers that peak for every batch size it can choose, down to a
+    budget that fits one system only."""
+    req = _req(0, 0.0, P=6, nmax=5, nl=16)
+    one = footprint(req.cluster, 1, "lu32ir")
+    for budget in (one, 3 * one, 16 * one, 64 * one):
+        s = Server(slots=1, budget=budget, solve_fn=lambda *a: None)
+        b, nbytes = s.plan(req)
+        assert nbytes <= budget
+        assert nbytes >= max(footprint(req.cluster, b, "lu32ir"), one)
+    for kind in ("lu64", "lu32ir"):
+        for b in (1, 2, 7, 32):
+            assert footprint(req.cluster, b, kind) >= footprint(req.cluster, 1, kind)
+
+
 @pytest.mark.gpu
 def test_results_same_alone_and_in_the_mix():
     """Each request solved alone (one slot, one at a time) and in a mix of
