@@ -196,6 +196,22 @@ int omx_trim(omx_context *ctx);
  * device and stream. */
 int omx_dense_solve(omx_context *ctx, int N, const double *A, const double *b, double *x, int kind);
 
+/* Allocation-free variant of omx_dense_solve with OMX_SOLVER_LU64 and
+ * OMX_PIVOT_PARTIAL (the plain FP64 LU with partial pivoting).  It has no
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Inputs and outputs are DEVICE pointers: A is an lda x Np
+ * column-major complex128 array with Np = N rounded up to 64 and lda >= Np,
+ * whose leading N x N block holds the matrix; the call overwrites A with the
+ * identity-padded matrix's LU factors.  b and x hold N complex128 values.
+ * Every work array (pivots, permutations, flags, diagonal-block inverses,
+ * staged right-hand side) lives in one library-held scratch array per device
+ * (about 8.6 MiB, held until the process ends), so N is limited to
+ * 1 <= N <= 4096 (OMX_ERR_ARG beyond).  Calls that overlap on one device share
+ * that array and race: issue them from one host thread at a time.  The call
+ * runs on the context's stream and returns after x is written. */
+int omx_dense_solve_noalloc(omx_context *ctx, int N, double *A, size_t lda, const double *b, double *x);
+
 #ifdef __cplusplus
 }
 #endif

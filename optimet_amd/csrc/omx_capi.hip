@@ -10,6 +10,7 @@
 #include <functional>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -132,6 +133,7 @@ struct omx_context {
         act;
   } sw;
   omx_stats stats{};
+  void *scratch = nullptr;  // this device's address of the no-allocation solve's scratch array
   long long mv_calls = 0;  // FP64 matrix-vector products since the last omx_set_profile
   double mv_ms = 0;
   double omega() const { return omx::kC0 * (2.0 * omx::kPi / lambda); }
@@ -1009,6 +1011,62 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
     }
 }
 
+// ------------------------------------------------------------------ no-allocation solve
+// omx_dense_solve_noalloc keeps every work array of its factorisation and
+// triangular solves in one file-scope device array, so a call allocates
+// nothing.  Each device holds its own copy for the life of the process; the
+// host learns that copy's address once per device (hipGetSymbolAddress in
+// omx_create, with the device current) and passes that address to kernels:
+// the array's name in host code is not a device address.
+constexpr int kNoAllocMaxN = 4096;  // largest order (and padded order) served
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+struct NoAllocLayout {
+  static constexpr int Np = kNoAllocMaxN;
+  static constexpr int gmax = Np / omx::kPanelRows + 1;
+  static constexpr size_t status = 0;  // 16 ints
+  static constexpr size_t flags = status + align256(16 * sizeof(int));  // Np / 64 + 1 ints
+  static constexpr size_t piv = flags + align256((Np / omx::kNB + 1) * sizeof(int));  // Np ints
+  static constexpr size_t perm = piv + align256(Np * sizeof(int));  // Np ints
+  static constexpr size_t cand = perm + align256(Np * sizeof(int));  // 2 x gmax x 64 ints
+  static constexpr size_t cnt = cand + align256(2 * gmax * omx::kNB * sizeof(int));  // 2 x gmax ints
+  static constexpr size_t map = cnt + align256(2 * gmax * sizeof(int));  // 3 x (Np + 128) ints
+  static constexpr size_t wlu = map + align256(3 * (Np + 2 * omx::kNB) * sizeof(int));  // 3 x 64 x 64 complex
+  static constexpr size_t dinv = wlu + align256(3 * omx::kNB * omx::kNB * sizeof(cplx));  // 2 x 64 x Np complex
+  static constexpr size_t rhs = dinv + align256(2 * omx::kNB * (size_t)Np * sizeof(cplx));  // Np complex
+  static constexpr size_t x = rhs + align256(Np * sizeof(cplx));  // Np complex
+  static constexpr size_t bytes = x + align256(Np * sizeof(cplx));
+};
+static_assert(NoAllocLayout::bytes < (9u << 20), "no-allocation scratch stays under 9 MiB");
+
+static __device__ __align__(256) unsigned char scratch[NoAllocLayout::bytes];
+
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_addr;  // per device, filled once
+
+// Device address of this device's scratch array (the device must be current).
+void *scratch_address(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if((int)g_scratch_addr.size() <= device) g_scratch_addr.resize(device + 1, nullptr);
+  if(!g_scratch_addr[device]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if(!p) throw OmxError(OMX_ERR_DEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_addr[device] = p;
+  }
+  return g_scratch_addr[device];
+}
+
+// Clears n ints (a kernel rather than a memset, so the call also suits code
+// that records it into a graph).
+__global__ void zero_ints_kernel(int *p, int n) {
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+// rb[0, N) = b, rb[N, Np) = 0: the right-hand side of the padded system.
+__global__ void stage_rhs_kernel(const cplx *b, cplx *rb, int N, int Np) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < Np) rb[i] = i < N ? b[i] : omx::mk(0.0);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1039,6 +1097,7 @@ int omx_create(omx_context **out, int device) {
       omx::fill_coef(h.data());
       HIPCHK(hipMemcpy(c->d_coef.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
       omx::upload_factorials();
+      c->scratch = scratch_address(device);
     } catch(...) {
       omx_destroy(c);
       throw;
@@ -1648,6 +1707,71 @@ int omx_trim(omx_context *c) {
     for(DBuf *b : big) b->release(&c->total);
     c->updated = c->solved = false;
     c->factored = c->factored_sh = false;
+  });
+}
+
+int omx_dense_solve_noalloc(omx_context *c, int N, double *A, size_t lda, const double *b, double *x) {
+  return guard([&] {
+    if(!c || !A || !b || !x) throw OmxError(OMX_ERR_ARG, "omx_dense_solve_noalloc: null argument");
+    if(N < 1 || N > kNoAllocMaxN)
+      throw OmxError(OMX_ERR_ARG, "omx_dense_solve_noalloc: N = " + std::to_string(N) + " is outside [1, " +
+                                      std::to_string(kNoAllocMaxN) + "], the limit of its scratch array");
+    const int Np = (N + omx::kNB - 1) / omx::kNB * omx::kNB;
+    if(lda < (size_t)Np)
+      throw OmxError(OMX_ERR_ARG, "omx_dense_solve_noalloc: lda = " + std::to_string(lda) +
+                                      " is below N rounded up to 64 (" + std::to_string(Np) + ")");
+    if(!c->scratch) throw OmxError(OMX_ERR_STATE, "omx_dense_solve_noalloc: context has no scratch address");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = c->st;
+    using L = NoAllocLayout;
+    unsigned char *base = static_cast<unsigned char *>(c->scratch);
+    auto ip = [&](size_t off) { return reinterpret_cast<int *>(base + off); };
+    omx::LuWork w{};
+    w.status = ip(L::status);
+    w.flags = ip(L::flags);
+    w.piv = ip(L::piv);
+    w.perm = ip(L::perm);
+    w.cand[0] = ip(L::cand);
+    w.cand[1] = ip(L::cand) + L::gmax * omx::kNB;
+    w.cnt[0] = ip(L::cnt);
+    w.cnt[1] = ip(L::cnt) + L::gmax;
+    w.mdst = ip(L::map);
+    w.msrc = ip(L::map) + (L::Np + 2 * omx::kNB);
+    w.mcnt = ip(L::map) + 2 * (L::Np + 2 * omx::kNB);
+    w.wlu = base + L::wlu;
+    w.dinv = base + L::dinv;
+    w.aux = c->aux;
+    w.ev_next = c->lu_ev[0];
+    w.ev_piv = c->lu_ev[1];
+    w.ev_bulk = c->lu_ev[2];
+    w.ev_pan = c->lu_ev[3];
+    w.gmax = L::gmax;
+    w.epoch = 0;
+    w.pivot = omx::kPivotPartial;
+    w.nb = 1;
+    cplx *rb = reinterpret_cast<cplx *>(base + L::rhs), *xs = reinterpret_cast<cplx *>(base + L::x);
+    // the array keeps the last call's contents: the status words and the
+    // triangular solves' flags (compared against epochs 1 and 2 below) start at zero
+    const int nz = (int)((L::piv - L::status) / sizeof(int));
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((nz + 255) / 256), dim3(256), 0, st, ip(L::status), nz);
+    hipLaunchKernelGGL(stage_rhs_kernel, dim3((Np + 255) / 256), dim3(256), 0, st, reinterpret_cast<const cplx *>(b),
+                       rb, N, Np);
+    omx::run_pad64(reinterpret_cast<cplx *>(A), N, Np, lda, st);
+    HIPCHK(hipGetLastError());
+    const int rc = omx::lu_factor<double2>(reinterpret_cast<double2 *>(A), Np, lda, w, st, nullptr);
+    if(rc) throw OmxError(OMX_ERR_DEVICE, "lu_factor launch failed (" + std::to_string(rc) + ")");
+    auto status = [&]() {
+      int s[2] = {0, 0};
+      HIPCHK(hipMemcpyAsync(s, w.status, sizeof(s), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      if(s[1]) throw OmxError(OMX_ERR_DEVICE, "device wait timed out (code " + std::to_string(s[1]) + ")");
+      if(s[0]) throw OmxError(OMX_ERR_SINGULAR, "zero pivot at column " + std::to_string(s[0] - 1));
+    };
+    status();
+    omx::lu_solve<double2>(reinterpret_cast<const double2 *>(A), Np, lda, w, rb, xs, st);
+    HIPCHK(hipMemcpyAsync(x, xs, sizeof(cplx) * N, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipGetLastError());
+    status();
   });
 }
 

@@ -27,7 +27,8 @@ ERRORS = {-1: "argument", -2: "device", -3: "memory", -4: "singular", -5: "state
 EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "omx_set_excitation",
            "omx_set_solver", "omx_set_cg_tables", "omx_build_cg_tables", "omx_update", "omx_solve",
            "omx_cross_sections", "omx_scattering_size", "omx_get_stats", "omx_read_matrix", "omx_read_vector",
-           "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_set_lu32_gemm", "omx_solve_sweep", "omx_mem_info", "omx_trim"]
+           "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_set_lu32_gemm", "omx_solve_sweep", "omx_mem_info", "omx_trim",
+           "omx_dense_solve_noalloc"]
 
 
 class OmxError(RuntimeError):
@@ -87,6 +88,7 @@ def load_library(path=LIB_PATH):
     SZ = ctypes.POINTER(ctypes.c_size_t)
     L.omx_mem_info.argtypes = [P, SZ, SZ, SZ]
     L.omx_trim.argtypes = [P]
+    L.omx_dense_solve_noalloc.argtypes = [P, I, V, ctypes.c_size_t, V, V]
     _lib = L
     return L
 
@@ -168,6 +170,12 @@ class Context:
         _check(self.L.omx_dense_solve(self.h, N, _d(flat.view(np.float64)), _d(bb.view(np.float64)),
                                       _d(x.view(np.float64)), KINDS[kind]))
         return x
+
+    def dense_solve_noalloc(self, N, A_dev, lda, b_dev, x_dev):
+        """omx_dense_solve_noalloc on device addresses (ints): A (lda x Np,
+        column-major, overwritten by its factors), b and x (N complex each)."""
+        _check(self.L.omx_dense_solve_noalloc(self.h, int(N), ctypes.c_void_p(A_dev), int(lda),
+                                              ctypes.c_void_p(b_dev), ctypes.c_void_p(x_dev)))
 
 
 class Solver:
