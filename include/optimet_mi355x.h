@@ -167,6 +167,10 @@ int omx_set_profile(omx_context *ctx, int on);
  * field of every object) to host memory. */
 int omx_read_matrix(omx_context *ctx, int which, double *out);
 int omx_read_vector(omx_context *ctx, int which, double *out);
+/* Parity access to one block of the assembled matrix (which 0: S, 1: V):
+ * rows row0 .. row0+nrows-1 and columns col0 .. col0+ncols-1, copied to out
+ * as an nrows x ncols column-major complex128 array (no full-matrix copy). */
+int omx_read_matrix_block(omx_context *ctx, int which, int row0, int col0, int nrows, int ncols, double *out);
 
 /* Wavelength sweep, batched: the current geometry and excitation solved at
  * nl vacuum wavelengths (m), as Simulation::scan_wavelengths does one

@@ -11,7 +11,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liboptimet_mi355x.so")
-SOURCES = ["omx_assemble.hip", "omx_lu.hip", "omx_iter.hip", "omx_sh.hip", "omx_capi.hip"]
+SOURCES = ["omx_assemble.hip", "omx_incident.hip", "omx_lu.hip", "omx_iter.hip", "omx_sh.hip", "omx_capi.hip"]
+# Per-file flags: the plane-wave coefficients reproduce the reference's
+# double arithmetic operation by operation (no fused multiply-adds).
+EXTRA = {"omx_incident.hip": ["-ffp-contract=off"]}
 ARCH = os.environ.get("OMX_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed-FP32 instructions stay disabled: the compiler would otherwise form
@@ -24,7 +27,7 @@ COMMON = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unused-r
 
 def _compile(src):
     obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *COMMON, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *COMMON, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stdout}\n{r.stderr}")

@@ -1488,6 +1488,21 @@ int omx_read_matrix(omx_context *c, int which, double *out) {
   });
 }
 
+int omx_read_matrix_block(omx_context *c, int which, int row0, int col0, int nrows, int ncols, double *out) {
+  return guard([&] {
+    if(!c || !c->updated || !out) throw OmxError(OMX_ERR_STATE, "no assembled matrix");
+    if(row0 < 0 || col0 < 0 || nrows < 1 || ncols < 1 || row0 + nrows > c->N || col0 + ncols > c->N)
+      throw OmxError(OMX_ERR_ARG, "block outside the N x N matrix");
+    const DBuf &b = which == 0 ? c->d_S64 : c->d_V64;
+    if(!b.p) throw OmxError(OMX_ERR_STATE, "matrix not held in FP64");
+    HIPCHK(hipSetDevice(c->device));
+    const cplx *src = b.as<cplx>() + (size_t)col0 * c->Np + row0;
+    HIPCHK(hipMemcpy2DAsync(out, sizeof(cplx) * nrows, src, sizeof(cplx) * c->Np, sizeof(cplx) * nrows, ncols,
+                            hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
 int omx_read_vector(omx_context *c, int which, double *out) {
   return guard([&] {
     if(!c || !c->updated || !out) throw OmxError(OMX_ERR_STATE, "nothing assembled");

@@ -28,7 +28,7 @@ EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "o
            "omx_set_solver", "omx_set_cg_tables", "omx_build_cg_tables", "omx_update", "omx_solve",
            "omx_cross_sections", "omx_scattering_size", "omx_get_stats", "omx_read_matrix", "omx_read_vector",
            "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_set_lu32_gemm", "omx_solve_sweep", "omx_mem_info", "omx_trim",
-           "omx_dense_solve_noalloc"]
+           "omx_dense_solve_noalloc", "omx_read_matrix_block"]
 
 
 class OmxError(RuntimeError):
@@ -89,6 +89,7 @@ def load_library(path=LIB_PATH):
     L.omx_mem_info.argtypes = [P, SZ, SZ, SZ]
     L.omx_trim.argtypes = [P]
     L.omx_dense_solve_noalloc.argtypes = [P, I, V, ctypes.c_size_t, V, V]
+    L.omx_read_matrix_block.argtypes = [P, I, I, I, I, I, D]
     _lib = L
     return L
 
@@ -270,6 +271,14 @@ class Solver:
         out = np.zeros(N * N, np.complex128)
         _check(self.ctx.L.omx_read_matrix(self.ctx.h, 1 if sh else 0, _d(out.view(np.float64))))
         return out.reshape(N, N).T
+
+    def matrix_block(self, i, j, sh=False):
+        """Block (row object i, column object j) of S (or V): 2nf x 2nf."""
+        n2 = 2 * self.cluster.nf
+        out = np.zeros(n2 * n2, np.complex128)
+        _check(self.ctx.L.omx_read_matrix_block(self.ctx.h, 1 if sh else 0, i * n2, j * n2, n2, n2,
+                                                _d(out.view(np.float64))))
+        return out.reshape(n2, n2).T
 
     def vector(self, which):
         """which: "Q", "K", "K1", "inc"."""

@@ -71,6 +71,8 @@ def _declare(L):
     L.ora_source_vector.argtypes = [_P, _D]
     L.ora_scattering_matrix.argtypes = [_P, _I, _D]
     L.ora_dense_solve.argtypes = [_I, _D, _D, _D, _I]
+    L.ora_scattering_blocks.argtypes = [_P, _I, _I, ctypes.POINTER(ctypes.c_int), _D]
+    L.ora_residual.argtypes = [_P, _I, _D, _D, _D]
     L.ora_cg_tables.argtypes = [_I, _I, _D]
     L.ora_wigner3j.argtypes = [_I] * 6
     L.ora_wigner3j.restype = _Dbl
@@ -212,6 +214,31 @@ def scattering_matrix(prob, sh=False):
     p = _Pinned(prob)
     _check(lib().ora_scattering_matrix(p.ref(), int(sh), _ptr(out.view(np.float64))))
     return out.reshape(N, N).T  # stored column-major
+
+
+def scattering_blocks(prob, pairs, sh=False):
+    """Blocks (i, j) of S (or V) for the given object pairs: array (n, 2nf, 2nf),
+    row index first, as scattering_matrix(prob)[i*2nf:(i+1)*2nf, j*2nf:(j+1)*2nf]."""
+    nf = prob.nmax * (prob.nmax + 2)
+    ij = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+    n = ij.shape[0]
+    out = _cz(n * 4 * nf * nf)
+    p = _Pinned(prob)
+    _check(lib().ora_scattering_blocks(p.ref(), int(sh), n, ij.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       _ptr(out.view(np.float64))))
+    return out.reshape(n, 2 * nf, 2 * nf).transpose(0, 2, 1)  # each block stored column-major
+
+
+def residual(prob, x, b, sh=False):
+    """b - S x (or b - V x) with the oracle's matrix assembled block row by block row."""
+    N = prob.N
+    xx = np.ascontiguousarray(x, dtype=np.complex128)
+    bb = np.ascontiguousarray(b, dtype=np.complex128)
+    r = _cz(N)
+    p = _Pinned(prob)
+    _check(lib().ora_residual(p.ref(), int(sh), _ptr(xx.view(np.float64)), _ptr(bb.view(np.float64)),
+                              _ptr(r.view(np.float64))))
+    return r
 
 
 def dense_solve(S, b, method=0):

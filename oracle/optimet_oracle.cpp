@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <exception>
 #include <complex>
 #include <cstring>
 #include <stdexcept>
@@ -46,6 +47,23 @@ struct OraError : std::runtime_error {
 };
 
 inline cd C(const double *p, int i) { return cd(p[2 * i], p[2 * i + 1]); }
+
+// Runs body(i) for i in [0, n) on all OpenMP threads; the first exception
+// thrown by any iteration is rethrown after the loop (none may escape a thread).
+template <typename F> void parallel_for(int n, F &&body) {
+  std::exception_ptr err = nullptr;
+#pragma omp parallel for schedule(dynamic, 1)
+  for(int i = 0; i < n; ++i) {
+    try {
+      body(i);
+    } catch(...) {
+#pragma omp critical(ora_err)
+      if(!err) err = std::current_exception();
+    }
+  }
+  if(err) std::rethrow_exception(err);
+}
+
 inline void put(double *p, size_t i, cd v) {
   p[2 * i] = v.real();
   p[2 * i + 1] = v.imag();
@@ -611,32 +629,39 @@ std::vector<cd> source_vector(const Problem &pb) {  // PreconditionedMatrix.cpp:
   return Q;
 }
 
-// preconditioned_scattering_matrix(SH) (PreconditionedMatrix.cpp:350-400, 555-610)
-// S_ij(r, c) = -T_i[r] * [[A^T, B^T],[B^T, A^T]](r, c), S_ii = I; column-major.
+// Block (row object i, column object j) of S (sh = false) or V (sh = true),
+// 2nf x 2nf column-major with leading dimension ld:
+// S_ij(r, c) = -T_i[r] * [[A^T, B^T],[B^T, A^T]](r, c), S_ii = I
+// (preconditioned_scattering_matrix(SH), PreconditionedMatrix.cpp:350-400, 555-610).
+void scattering_block(const Problem &pb, bool sh, int i, int j, const std::vector<cd> &T, cd *out, size_t ld) {
+  int nf = pb.nf;
+  if(i == j) {
+    for(int c = 0; c < 2 * nf; ++c)
+      for(int r = 0; r < 2 * nf; ++r) out[(size_t)c * ld + r] = r == c ? 1.0 : 0.0;
+    return;
+  }
+  cd k = sh ? 2.0 * pb.waveK : pb.waveK;
+  std::vector<cd> A, B;
+  coupling(sph_minus(pb.obj[i].vR, pb.obj[j].vR), k, pb.nmax, true, A, B);
+  for(int c = 0; c < 2 * nf; ++c)
+    for(int r = 0; r < 2 * nf; ++r) {
+      int rr = r % nf, cc = c % nf;
+      bool diag = (r < nf) == (c < nf);
+      cd blk = diag ? A[(size_t)cc * nf + rr] : B[(size_t)cc * nf + rr];
+      out[(size_t)c * ld + r] = -T[r] * blk;
+    }
+}
+
+// The whole N x N matrix, column-major; block rows on all threads.
 void scattering_matrix(const Problem &pb, bool sh, std::vector<cd> &S) {
   int nf = pb.nf, P = pb.P;
   size_t N = (size_t)2 * nf * P;
   S.assign(N * N, cd(0, 0));
-  cd k = sh ? 2.0 * pb.waveK : pb.waveK;
-  for(int i = 0; i < P; ++i) {
+  parallel_for(P, [&](int i) {
     auto T = object_vector(pb, pb.obj[i], sh ? 1 : 0, pb.bg_eps, pb.bg_mu);
-    for(int j = 0; j < P; ++j) {
-      size_t x = (size_t)i * 2 * nf, y = (size_t)j * 2 * nf;
-      if(i == j) {
-        for(int r = 0; r < 2 * nf; ++r) S[(y + r) * N + x + r] = 1.0;
-        continue;
-      }
-      std::vector<cd> A, B;
-      coupling(sph_minus(pb.obj[i].vR, pb.obj[j].vR), k, pb.nmax, true, A, B);
-      for(int c = 0; c < 2 * nf; ++c)
-        for(int r = 0; r < 2 * nf; ++r) {
-          int rr = r % nf, cc = c % nf;
-          bool diag = (r < nf) == (c < nf);
-          cd blk = diag ? A[(size_t)cc * nf + rr] : B[(size_t)cc * nf + rr];
-          S[(y + c) * N + x + r] = -T[r] * blk;
-        }
-    }
-  }
+    for(int j = 0; j < P; ++j)
+      scattering_block(pb, sh, i, j, T, &S[(size_t)j * 2 * nf * N + (size_t)i * 2 * nf], N);
+  });
 }
 
 // ---------------------------------------------------------------- solves
@@ -748,12 +773,12 @@ void dense_solve(int N, const std::vector<cd> &S, const std::vector<cd> &b, std:
 
 // ---------------------------------------------------------------- Wigner
 long double fact(int n) {
-  static std::vector<long double> t;
-  if(t.empty()) {
-    t.resize(171);
-    t[0] = 1;
-    for(int i = 1; i < 171; ++i) t[i] = t[i - 1] * i;
-  }
+  static const std::vector<long double> t = [] {  // initialised once, thread-safe
+    std::vector<long double> v(171);
+    v[0] = 1;
+    for(int i = 1; i < 171; ++i) v[i] = v[i - 1] * i;
+    return v;
+  }();
   if(n < 0) throw OraError("negative factorial");
   return t[n];
 }
@@ -822,9 +847,9 @@ void cg_tables(int nmax, int nmaxs, double *out) {
   int nf = nmax * (nmax + 2), nfs = nmaxs * (nmaxs + 2);
   size_t sz = (size_t)nfs * nf * nf;
   const double s32 = std::sqrt(3.0 / 2.0 / kPi);
-  size_t idx = 0;
-  for(int kk = 0; kk < nfs; ++kk) {
+  parallel_for(nfs, [&](int kk) {  // output harmonics on all threads
     int J = (int)std::sqrt(kk + 1.0), M = -(kk + 1) + J * (J + 1);
+    size_t idx = (size_t)kk * nf * nf;
     for(int p = 0; p < nf; ++p) {
       int J1 = (int)std::sqrt(p + 1.0), M1 = -(p + 1) + J1 * (J1 + 1);
       for(int q = 0; q < nf; ++q, ++idx) {
@@ -892,7 +917,7 @@ void cg_tables(int nmax, int nmaxs, double *out) {
         out[8 * sz + idx] = b2 * Wfun(J1, J1, M1, J2 - 1, J2, M2, J, M) + a2 * Wfun(J1, J1, M1, J2 + 1, J2, M2, J, M);
       }
     }
-  }
+  });
 }
 
 // ---------------------------------------------------------------- SH source
@@ -948,8 +973,8 @@ void source_sh(const Problem &pb, const double *tab, const std::vector<cd> &x_in
   for(size_t i = 0; i < u.size(); ++i) u[i] = std::conj(x_int[i]);
   K.assign((size_t)2 * nf * pb.P, 0.0);
   K1.assign((size_t)2 * nf * pb.P, 0.0);
-  std::vector<cd> ana(4 * nf);
-  for(int j = 0; j < pb.P; ++j) {
+  parallel_for(pb.P, [&](int j) {
+    std::vector<cd> ana(4 * nf);
     inc_local_sh(pb, tab, u, j, ana.data());
     auto t1 = object_vector(pb, pb.obj[j], 5, pb.bg_eps, pb.bg_mu);
     auto t2 = object_vector(pb, pb.obj[j], 6, pb.bg_eps, pb.bg_mu);
@@ -958,7 +983,7 @@ void source_sh(const Problem &pb, const double *tab, const std::vector<cd> &x_in
       K[(size_t)j * 2 * nf + r] = t1[r] * ana[r] + t2[r] * ana[2 * nf + r];
       K1[(size_t)j * 2 * nf + r] = ia2[r] * ana[2 * nf + r];
     }
-  }
+  });
 }
 
 // ---------------------------------------------------------------- cross sections
@@ -1124,6 +1149,45 @@ int ora_scattering_matrix(const ora_problem *in, int sh, double *S) {
     std::vector<cd> s;
     scattering_matrix(pb, sh != 0, s);
     std::memcpy(S, s.data(), s.size() * sizeof(cd));
+  });
+}
+
+int ora_scattering_blocks(const ora_problem *in, int sh, int nblk, const int *ij, double *out) {
+  return guard([&] {
+    Problem pb = make_problem(in);
+    const int nf = pb.nf;
+    const size_t bs = (size_t)4 * nf * nf;
+    for(int b = 0; b < nblk; ++b)
+      if(ij[2 * b] < 0 || ij[2 * b] >= pb.P || ij[2 * b + 1] < 0 || ij[2 * b + 1] >= pb.P)
+        throw OraError("block index outside the cluster");
+    parallel_for(nblk, [&](int b) {
+      const int i = ij[2 * b], j = ij[2 * b + 1];
+      auto T = object_vector(pb, pb.obj[i], sh ? 1 : 0, pb.bg_eps, pb.bg_mu);
+      scattering_block(pb, sh != 0, i, j, T, reinterpret_cast<cd *>(out) + b * bs, (size_t)2 * nf);
+    });
+  });
+}
+
+int ora_residual(const ora_problem *in, int sh, const double *x, const double *b, double *r) {
+  return guard([&] {
+    Problem pb = make_problem(in);
+    const int nf = pb.nf, P = pb.P, n2 = 2 * nf;
+    const size_t N = (size_t)n2 * P;
+    const cd *xx = reinterpret_cast<const cd *>(x), *bb = reinterpret_cast<const cd *>(b);
+    cd *rr = reinterpret_cast<cd *>(r);
+    parallel_for(P, [&](int i) {
+      auto T = object_vector(pb, pb.obj[i], sh ? 1 : 0, pb.bg_eps, pb.bg_mu);
+      std::vector<cd> blk((size_t)n2 * n2);
+      std::vector<cd> acc(bb + (size_t)i * n2, bb + (size_t)(i + 1) * n2);
+      for(int j = 0; j < P; ++j) {
+        scattering_block(pb, sh != 0, i, j, T, blk.data(), n2);
+        const cd *xj = xx + (size_t)j * n2;
+        for(int c = 0; c < n2; ++c)
+          for(int q = 0; q < n2; ++q) acc[q] -= blk[(size_t)c * n2 + q] * xj[c];
+      }
+      std::copy(acc.begin(), acc.end(), rr + (size_t)i * n2);
+    });
+    (void)N;
   });
 }
 

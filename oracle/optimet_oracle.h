@@ -79,6 +79,15 @@ int ora_source_vector(const ora_problem *pb, double *Q);
 /* S (sh=0) or V (sh=1): N*N complex, column-major. */
 int ora_scattering_matrix(const ora_problem *pb, int sh, double *S);
 
+/* Blocks (i, j) of S (sh=0) or V (sh=1) for the nblk object pairs ij[2b], ij[2b+1]:
+ * out holds nblk blocks of 2nf x 2nf complex, each column-major.  Pairs run on all
+ * OpenMP threads. */
+int ora_scattering_blocks(const ora_problem *pb, int sh, int nblk, const int *ij, double *out);
+
+/* r = b - S x (sh=0) or b - V x (sh=1), N complex each, with the matrix assembled
+ * one block row at a time (never held whole); block rows run on all OpenMP threads. */
+int ora_residual(const ora_problem *pb, int sh, const double *x, const double *b, double *r);
+
 /* Dense solve of S x = b (column-major, N*N). method 0: LU with partial
  * pivoting; 1: column-pivoted Householder QR. */
 int ora_dense_solve(int N, const double *S, const double *b, double *x, int method);

@@ -27,13 +27,20 @@ def small_drude(P=5, nmax=4, sh=True, seed=11):
 
 
 def check_internal(ora, c, out, ref, tol):
-    """x_int = x_sca (.) Iaux (Solver.cpp:57-77).  Iaux grows like (2n+1)!!/x^n
-    with the harmonic order, so it weights the smallest scattered coefficients
-    most: the unpreconditioning step itself is pinned exactly, and x_int is held
-    to the coefficient bar times the largest relative weight of Iaux (<= 100)."""
+    """x_int = x_sca (.) Iaux (Solver.cpp:57-77).  The unpreconditioning step
+    itself is pinned: x_int equals x_sca (.) Iaux_oracle to 1e-12.  x_int
+    against the oracle then obeys the bound that the elementwise product
+    fixes, ||dx_int|| / ||x_int|| <= amp ||dx_sca|| / ||x_sca|| (+ the 1e-12 of
+    the Iaux pin), amp = max|Iaux| ||x_sca|| / ||x_int||: Iaux grows like
+    (2n+1)!!/x^n with the order, so amp reaches 1e15 for plasmonic spheres at
+    nmax 8, where two FP64 solvers of the same system differ by O(1) on x_int
+    (test_oracle_pins.py::test_internal_coefficients_are_ill_conditioned)."""
     iaux = np.concatenate([ora.object_vector(c, j, ora.OBJ_IAUX) for j in range(c.nobj)])
     assert rel(out["x_int"], out["x_sca"] * iaux) < 1e-12
-    assert rel(out["x_int"], ref["x_int"]) < 100 * tol
+    amp = np.max(np.abs(iaux)) * np.linalg.norm(ref["x_sca"]) / np.linalg.norm(ref["x_int"])
+    e = rel(out["x_sca"], ref["x_sca"])
+    assert e < tol
+    assert rel(out["x_int"], ref["x_int"]) <= amp * e * (1 + 1e-6) + 1e-12
 
 
 def make(ctx, c, kind="lu64", pivoting="block"):
@@ -110,6 +117,24 @@ def test_second_harmonic_chain(ctx, ora, kind, tol):
     assert abs(cs["csca_sh"] - ref["csca_sh"]) <= tol * abs(ref["csca_sh"])
 
 
+@pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL)])
+def test_nmax9_cluster(ctx, ora, kind, tol):
+    """nmax 9 (pair_kernel<9>): three Drude spheres, omega and 2 omega; S, V
+    and Q to 1e-12, x_sca, x_sca_sh and the cross sections at the bar."""
+    c = drude_cluster(3, 9, 650e-9, 50, 80, 500, 10, seed=9)
+    s = make(ctx, c, kind)
+    assert rel(s.vector("Q"), ora.source_vector(c)) < 1e-12
+    assert rel(s.matrix(), ora.scattering_matrix(c)) < 1e-12
+    assert rel(s.matrix(sh=True), ora.scattering_matrix(c, sh=True)) < 1e-12
+    ref = ora.solve_all(c)
+    out = s.solve()
+    cs = s.cross_sections()
+    assert rel(out["x_sca"], ref["x_sca"]) < tol
+    assert rel(out["x_sca_sh"], ref["x_sca_sh"]) < tol
+    for key in ("cext", "csca", "cabs", "csca_sh"):
+        assert abs(cs[key] - ref[key]) <= tol * abs(ref[key]), key
+
+
 def test_caller_supplied_cg_tables(ctx, ora):
     c = small_drude(P=3, nmax=3, seed=5)
     ref = ora.solve_all(c)
@@ -156,7 +181,8 @@ def test_lu32_gemm_modes(ctx, N):
 
 
 def test_lu32_gemm_modes_cluster(ctx, ora):
-    """N = 768 cluster, mixed path with either product mode, against the oracle."""
+    """N = 768 cluster, mixed path with either product mode, against the oracle
+    (x_sca and x_sca_sh at the mixed bar)."""
     c = drude_cluster(8, 6, 750e-9, 40, 80, 800, 10, seed=21, sh=True)
     ref = ora.solve_all(c)
     try:
@@ -165,7 +191,7 @@ def test_lu32_gemm_modes_cluster(ctx, ora):
             s = make(ctx, c, "lu32ir")
             out = s.solve()
             assert rel(out["x_sca"], ref["x_sca"]) < MIXED_TOL, mode
-            assert rel(out["x_sca_sh"], ref["x_sca_sh"]) < 100 * MIXED_TOL, mode
+            assert rel(out["x_sca_sh"], ref["x_sca_sh"]) < MIXED_TOL, mode
     finally:
         ctx.set_lu32_gemm("bf16x3")
 
@@ -240,7 +266,7 @@ def test_partial_pivoting_mode(ctx, ora, kind, tol):
     out = s.solve()
     cs = s.cross_sections()
     assert rel(out["x_sca"], ref["x_sca"]) < tol
-    assert rel(out["x_sca_sh"], ref["x_sca_sh"]) < 100 * tol
+    assert rel(out["x_sca_sh"], ref["x_sca_sh"]) < tol
     assert abs(cs["cext"] - ref["cext"]) / abs(ref["cext"]) < tol
     ctx.set_pivoting("block")
 
@@ -290,8 +316,8 @@ def test_golden_fixtures_gpu(ctx, kind, tol):
         for key in ("cext", "csca", "cabs"):
             assert abs(cs[key] - ref[key]) <= tol * abs(ref[key]), (name, key)
         if "x_sca_sh" in ref.files:
-            assert rel(out["x_sca_sh"], ref["x_sca_sh"]) < 100 * tol, name
-            assert abs(cs["csca_sh"] - ref["csca_sh"]) <= 100 * tol * abs(ref["csca_sh"]), name
+            assert rel(out["x_sca_sh"], ref["x_sca_sh"]) < tol, name
+            assert abs(cs["csca_sh"] - ref["csca_sh"]) <= tol * abs(ref["csca_sh"]), name
 
 
 @pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL)])
@@ -309,11 +335,10 @@ def test_wavelength_sweep_batched(ctx, ora, kind, tol, P, nmax, batch):
     for i, lam in enumerate(lams):
         ref = ora.solve_all(c.with_wavelength(lam))
         assert rel(out["x_sca"][i], ref["x_sca"]) < tol, (i, lam)
-        assert rel(out["x_sca_sh"][i], ref["x_sca_sh"]) < 100 * tol, (i, lam)
+        assert rel(out["x_sca_sh"][i], ref["x_sca_sh"]) < tol, (i, lam)
         xs = out["xs"][i]
         for j, key in enumerate(("cext", "csca", "cabs", "csca_sh")):
-            t = tol if j < 3 else 100 * tol
-            assert abs(xs[j] - ref[key]) <= t * abs(ref[key]), (i, key)
+            assert abs(xs[j] - ref[key]) <= tol * abs(ref[key]), (i, key)
         assert out["iterations"][i, 0] > 0
 
 

@@ -401,3 +401,31 @@ def test_golden_fixtures(ora):
         for key in ref.files:
             got = np.asarray(out[key])
             assert np.linalg.norm(got - ref[key]) <= 1e-12 * np.linalg.norm(ref[key]), (name, key)
+
+
+def test_internal_coefficients_are_ill_conditioned(ora):
+    """x_int = x_sca (.) Iaux (Solver.cpp:57-77) cannot be held to a relative
+    bar of the coefficients: for six Drude spheres at nmax 8 (N = 960) two
+    backward-stable FP64 solvers of the same S (LAPACK's LU and a
+    column-pivoted Householder QR, the reference's algorithm) agree on x_sca to
+    about 2e-12 but differ by O(1) on x_int, because max|Iaux| (about 4e16)
+    amplifies the rounding of the tiny high-order x_sca entries.  The error of
+    x_int is bounded by amp * err(x_sca) with amp = max|Iaux| ||x_sca|| /
+    ||x_int||, the bound the GPU tests apply."""
+    import scipy.linalg as sl
+    from optimet_amd.problem import drude_cluster
+    c = drude_cluster(6, 8, 800e-9, 40, 80, 600, 10, seed=7, sh=False)
+    S, Q = ora.scattering_matrix(c), ora.source_vector(c)
+    x_lu = np.linalg.solve(S, Q)
+    q, r, p = sl.qr(S, pivoting=True)
+    x_qr = np.empty_like(x_lu)
+    x_qr[p] = sl.solve_triangular(r, q.conj().T @ Q)
+    ia = np.concatenate([ora.object_vector(c, j, ora.OBJ_IAUX) for j in range(c.nobj)])
+    e = np.linalg.norm(x_lu - x_qr) / np.linalg.norm(x_lu)
+    xi_lu, xi_qr = x_lu * ia, x_qr * ia
+    ei = np.linalg.norm(xi_lu - xi_qr) / np.linalg.norm(xi_lu)
+    amp = np.max(np.abs(ia)) * np.linalg.norm(x_lu) / np.linalg.norm(xi_lu)
+    assert e < 1e-10
+    assert np.max(np.abs(ia)) > 1e15
+    assert ei > 1e-2  # far above any coefficient bar
+    assert ei <= amp * e * (1 + 1e-6)
