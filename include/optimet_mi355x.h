@@ -15,6 +15,8 @@
  *   omx_cross_sections                      <- Result::getExtinctionCrossSection,
  *       getScatteringCrossSection, getAbsorptionCrossSection and
  *       getScatteringCrossSection_SH (Result.cpp:557-761);
+ *   omx_absorption_cross_section_sh         <- Result::getAbsorptionCrossSection_SH
+ *       (Result.cpp:763-794);
  *   omx_scattering_size                     <- AbstractSolver::scattering_size()
  *       (Solver.h:132-135).
  *
@@ -153,6 +155,12 @@ int omx_solve(omx_context *ctx, double *x_sca, double *x_int, double *x_sca_sh, 
  * conventions (the incoherent C_sca sum of Result.cpp:629-631 included). */
 int omx_cross_sections(omx_context *ctx, double *out4);
 
+/* C_abs_SH of the last solve with SH_cond (Result::getAbsorptionCrossSection_SH,
+ * Result.cpp:763-794, with AbsCSSHcoeff, Geometry.cpp:428-455): the radial
+ * Gauss-Legendre integral of |E(2 omega)|^2 inside each sphere times the SH
+ * conductivity -i eps0 2 omega (eps_r_SH - 1). */
+int omx_absorption_cross_section_sh(omx_context *ctx, double *out);
+
 /* 2 nmax (nmax+2) P, AbstractSolver::scattering_size(). */
 int omx_scattering_size(const omx_context *ctx);
 
@@ -182,8 +190,10 @@ int omx_read_matrix_block(omx_context *ctx, int which, int row0, int col0, int n
  * pivoting) and refined together; a system with a zero pivot, or still short
  * of the tolerance after four restart cycles of the lockstep refinement, is
  * redone alone with partial pivoting.  Outputs (any
- * may be NULL): x_sca, x_int, x_sca_sh, x_int_sh nl*N complex each, xs nl*4
- * cross sections as omx_cross_sections, iterations nl*2 (omega, 2 omega). */
+ * may be NULL): x_sca, x_int, x_sca_sh, x_int_sh nl*N complex each, xs nl*5
+ * cross sections per wavelength, [C_ext, C_sca, C_abs, C_sca_SH, C_abs_SH]
+ * (omx_cross_sections and omx_absorption_cross_section_sh; the scan writes
+ * these, Simulation.cpp:657-671), iterations nl*2 (omega, 2 omega). */
 int omx_solve_sweep(omx_context *ctx, int nl, const double *lambdas, const double *eps_r, const double *eps_r_sh,
                     int batch, double *x_sca, double *x_int, double *x_sca_sh, double *x_int_sh, double *xs,
                     int *iterations);

@@ -102,7 +102,7 @@ struct omx_context {
   int cg_nmax = 0;
   // device buffers
   DBuf d_xyz, d_radius, d_eps, d_mu, d_eps_sh, d_kppp, d_kpar, d_gam, d_sph, d_ov, d_inc, d_incloc, d_Q, d_w,
-      d_coef, d_S64, d_S32, d_V64, d_V32, d_x, d_xsh, d_xint, d_xintsh, d_K, d_K1, d_cross, d_cg;
+      d_coef, d_S64, d_S32, d_V64, d_V32, d_x, d_xsh, d_xint, d_xintsh, d_K, d_K1, d_cross, d_cg, d_abscoef, d_abssh;
   DBuf d_piv, d_perm, d_cand, d_cnt, d_map, d_status, d_flags, d_wlu;
   hipStream_t aux = nullptr;
   hipEvent_t lu_ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -128,7 +128,8 @@ struct omx_context {
   DBuf d_dinv, d_dinv2, d_F64, d_F64sh;  // diagonal-block inverses; FP64 factors (the FP64 matrices stay intact)
   // wavelength sweeps: per-system buffers of one batch ([system][...])
   struct Sweep {
-    DBuf S64, S32, V64, V32, F64, F64sh, eps, eps_sh, ov, incloc, Q, w, x, xint, xsh, xintsh, K, K1, cross;
+    DBuf S64, S32, V64, V32, F64, F64sh, eps, eps_sh, ov, incloc, Q, w, x, xint, xsh, xintsh, K, K1, cross, abscoef,
+        abssh;
     DBuf perm, perm2, flags, flags2, dinv, dinv2, map, status, basis, basisZ, r, w1, w2, gpart, part, hd, nrm, sc,
         act;
   } sw;
@@ -238,6 +239,8 @@ void alloc_all(omx_context *c) {
   c->d_K.ensure(sizeof(cplx) * Np, T);
   c->d_K1.ensure(sizeof(cplx) * Np, T);
   c->d_cross.ensure(sizeof(double) * 4 * P, T);
+  c->d_abscoef.ensure(sizeof(cplx) * P * nf, T);
+  c->d_abssh.ensure(sizeof(double) * P, T);
   c->d_r.ensure(sizeof(cplx) * Np, T);
   c->d_w1.ensure(sizeof(cplx) * Np, T);
   c->d_w2.ensure(sizeof(cplx) * Np, T);
@@ -793,6 +796,8 @@ void sweep_alloc(omx_context *c, int nb, bool f64, bool sh) {
     b->ensure(sizeof(cplx) * Np * nb, T);
   w.w.ensure(sizeof(double) * 2 * P * c->nf * nb, T);
   w.cross.ensure(sizeof(double) * 4 * P * nb, T);
+  w.abscoef.ensure(sizeof(cplx) * P * c->nf, T);
+  w.abssh.ensure(sizeof(double) * P * nb, T);
   for(DBuf *b : {&w.perm, &w.perm2}) b->ensure(sizeof(int) * Np * nb, T);
   for(DBuf *b : {&w.flags, &w.flags2}) b->ensure(sizeof(int) * (Np / omx::kNB + 1) * nb, T);
   for(DBuf *b : {&w.dinv, &w.dinv2}) b->ensure(sizeof(cplx) * omx::dinv_elems(Np) * nb, T);
@@ -984,8 +989,19 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
                       w.cross.as<double>() + (size_t)z * 4 * P};
     omx::run_cross(ca, st);
   }
-  std::vector<double> part((size_t)4 * P * nb);
+  // SH absorption (Result.cpp:763-794), per system; the scratch is reused in stream order
+  if(sh)
+    for(int z = 0; z < nb; ++z) {
+      const double omega = omx::kC0 * (2.0 * omx::kPi / lam[z]);
+      omx::AbsShArgs aa{P, nmax, omega, bge, bgm, c->d_radius.as<double>(), w.eps.as<cplx>() + (size_t)z * P,
+                        c->d_mu.as<cplx>(), w.eps_sh.as<cplx>() + (size_t)z * P, c->d_gam.as<cplx>(),
+                        c->d_cg.as<double>(), w.xint.as<cplx>() + z * lda, w.xintsh.as<cplx>() + z * lda,
+                        w.abscoef.as<cplx>(), w.abssh.as<double>() + (size_t)z * P};
+      omx::run_abssh(aa, st);
+    }
+  std::vector<double> part((size_t)4 * P * nb), pabs((size_t)P * nb, 0.0);
   HIPCHK(hipMemcpyAsync(part.data(), w.cross.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+  if(sh) HIPCHK(hipMemcpyAsync(pabs.data(), w.abssh.p, sizeof(double) * pabs.size(), hipMemcpyDeviceToHost, st));
   auto cp = [&](double *dst, const DBuf &b) {
     if(dst)
       HIPCHK(hipMemcpy2DAsync(dst, sizeof(cplx) * N, b.p, sizeof(cplx) * Np, sizeof(cplx) * N, nb,
@@ -1004,10 +1020,13 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
       for(int j = 0; j < P; ++j)
         for(int e = 0; e < 4; ++e) sm[e] += part[(size_t)z * 4 * P + 4 * j + e];
       const double rk = 2.0 * omx::kPi / lam[z] * bgn.real();
-      xs[4 * z + 0] = (-1. / (rk * rk)) * sm[0];  // Result.cpp:576
-      xs[4 * z + 1] = (1. / (rk * rk)) * sm[1];   // Result.cpp:646
-      xs[4 * z + 2] = (1. / (rk * rk)) * sm[2];   // Result.cpp:675
-      xs[4 * z + 3] = sh ? (1.0 / (4.0 * c->bg_eps_r.x * c->bg_mu_r.x)) * sm[3] : 0.0;  // Result.cpp:760
+      double sa = 0.0;
+      for(int j = 0; j < P; ++j) sa += pabs[(size_t)z * P + j];
+      xs[5 * z + 0] = (-1. / (rk * rk)) * sm[0];  // Result.cpp:576
+      xs[5 * z + 1] = (1. / (rk * rk)) * sm[1];   // Result.cpp:646
+      xs[5 * z + 2] = (1. / (rk * rk)) * sm[2];   // Result.cpp:675
+      xs[5 * z + 3] = sh ? (1.0 / (4.0 * c->bg_eps_r.x * c->bg_mu_r.x)) * sm[3] : 0.0;  // Result.cpp:760
+      xs[5 * z + 4] = sa;                                                               // Result.cpp:763-794
     }
 }
 
@@ -1119,13 +1138,13 @@ void omx_destroy(omx_context *c) {
                   &c->d_K1, &c->d_cross, &c->d_cg, &c->d_piv, &c->d_perm, &c->d_cand, &c->d_cnt, &c->d_map,
                   &c->d_wlu, &c->d_status, &c->d_flags, &c->d_basis, &c->d_basisZ, &c->d_w1, &c->d_w2, &c->d_r,
                   &c->d_part, &c->d_h, &c->d_nrm, &c->d_gpart, &c->d_piv2, &c->d_perm2, &c->d_flags2,
-                  &c->d_dinv, &c->d_dinv2, &c->d_F64, &c->d_F64sh};
+                  &c->d_dinv, &c->d_dinv2, &c->d_F64, &c->d_F64sh, &c->d_abscoef, &c->d_abssh};
   for(DBuf *b : bufs) b->release(&c->total);
   auto &w = c->sw;
   DBuf *sbufs[] = {&w.S64, &w.S32, &w.V64, &w.V32, &w.F64, &w.F64sh, &w.eps, &w.eps_sh, &w.ov, &w.incloc, &w.Q,
                    &w.w, &w.x, &w.xint, &w.xsh, &w.xintsh, &w.K, &w.K1, &w.cross, &w.perm, &w.perm2, &w.flags,
                    &w.flags2, &w.dinv, &w.dinv2, &w.map, &w.status, &w.basis, &w.basisZ, &w.r, &w.w1, &w.w2, &w.gpart,
-                   &w.part, &w.hd, &w.nrm, &w.sc, &w.act};
+                   &w.part, &w.hd, &w.nrm, &w.sc, &w.act, &w.abscoef, &w.abssh};
   for(DBuf *b : sbufs) b->release(&c->total);
   for(auto &e : c->ev)
     if(e) (void)hipEventDestroy(e);
@@ -1448,6 +1467,26 @@ int omx_cross_sections(omx_context *c, double *out4) {
   });
 }
 
+int omx_absorption_cross_section_sh(omx_context *c, double *out) {
+  return guard([&] {
+    if(!c || !c->solved || !out) throw OmxError(OMX_ERR_STATE, "omx_solve must precede omx_absorption_cross_section_sh");
+    if(!c->sh || c->cg_nmax != c->nmax) throw OmxError(OMX_ERR_STATE, "no second-harmonic solution");
+    HIPCHK(hipSetDevice(c->device));
+    omx::AbsShArgs aa{c->P, c->nmax, c->omega(), kEps0 * c->bg_eps_r, kMu0 * c->bg_mu_r, c->d_radius.as<double>(),
+                      c->d_eps.as<cplx>(), c->d_mu.as<cplx>(), c->d_eps_sh.as<cplx>(), c->d_gam.as<cplx>(),
+                      c->d_cg.as<double>(), c->d_xint.as<cplx>(), c->d_xintsh.as<cplx>(), c->d_abscoef.as<cplx>(),
+                      c->d_abssh.as<double>()};
+    omx::run_abssh(aa, c->st);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part(c->P);
+    HIPCHK(hipMemcpyAsync(part.data(), c->d_abssh.p, sizeof(double) * c->P, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double s = 0.0;
+    for(double v : part) s += v;
+    *out = s;
+  });
+}
+
 int omx_scattering_size(const omx_context *c) { return c ? c->N : 0; }
 
 int omx_get_stats(const omx_context *c, omx_stats *out) {
@@ -1621,7 +1660,7 @@ int omx_solve_sweep(omx_context *c, int nl, const double *lambdas, const double 
       std::vector<double> res(2 * n, 0.0);
       sweep_chunk(c, n, lambdas + l0, eps.data() + (size_t)l0 * P, eps_sh.data() + (size_t)l0 * P,
                   off(x_sca, l0, N), off(x_int, l0, N), off(x_sca_sh, l0, N), off(x_int_sh, l0, N),
-                  xs ? xs + 4 * l0 : nullptr, bad, its, res);
+                  xs ? xs + 5 * l0 : nullptr, bad, its, res);
       for(int z = 0; z < n; ++z) {
         if(bad[z]) redo.push_back(l0 + z);
         if(iterations) {
@@ -1664,8 +1703,13 @@ int omx_solve_sweep(omx_context *c, int nl, const double *lambdas, const double 
           rc = omx_solve(c, off(x_sca, l, N), off(x_int, l, N), off(x_sca_sh, l, N), off(x_int_sh, l, N));
           if(rc) throw OmxError(rc, omx_last_error());
           if(xs) {
-            rc = omx_cross_sections(c, xs + 4 * l);
+            rc = omx_cross_sections(c, xs + 5 * l);
             if(rc) throw OmxError(rc, omx_last_error());
+            xs[5 * l + 4] = 0.0;
+            if(c->sh) {
+              rc = omx_absorption_cross_section_sh(c, xs + 5 * l + 4);
+              if(rc) throw OmxError(rc, omx_last_error());
+            }
           }
           if(iterations) {
             iterations[2 * l] = c->stats.iterations;

@@ -20,6 +20,8 @@ constexpr int kMaxNmax = 10;                 // largest harmonic order supported
 constexpr int kMaxL = 2 * kMaxNmax + 1;      // recurrence domain l <= 2 nmax (+1 guard)
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kC0 = 299792458.0;
+constexpr double kMu0 = 4.0 * kPi * 1e-7;           // constants.cpp:19-22
+constexpr double kEps0 = 1.0 / (kMu0 * kC0 * kC0);
 constexpr double kErrEps = 1e-10;
 
 struct cplx {

@@ -8,6 +8,10 @@
 //                  (Geometry.cpp:250-310; Symbol.cpp:155-353) for every
 //                  object from conj(x_int), followed by source_vectorSH and
 //                  source_vectorSH_K1ana (PreconditionedMatrix.cpp:1347-1436)
+//   abssh_kernel   the SH absorption integrand of symbol::ACSshcoeff
+//                  (Symbol.cpp:358-470) per object and output harmonic,
+//                  reduced per object by abssh_sum_kernel into
+//                  Result::getAbsorptionCrossSection_SH (Result.cpp:763-794)
 #include "omx_kernels.hpp"
 #include "omx_sh.hpp"
 
@@ -193,6 +197,127 @@ __global__ __launch_bounds__(256) void shsrc_kernel(ShArgs a) {
   }
 }
 
+// One workgroup per (object j, output harmonic kk): the 4-point
+// Gauss-Legendre radial integral over [0, R] of |E(2 omega)|^2, the field of
+// the bulk (gamma) source of the fundamental internal field (x_int,
+// unconjugated) plus the internal SH field (x_int_sh).  Threads take the
+// (p, q) pairs of the bilinear sums for all four radii at once; the Bessel
+// arrays of the four radii sit in LDS.  The reference's literal forms are
+// kept: the second derivative of bessel3der (Bessel.h:230-231, i in place of
+// i + 1) and the SH X(+1) term j_n / r + j_n'(K r) without the factor K
+// (Symbol.cpp:460).
+__global__ __launch_bounds__(256) void abssh_kernel(AbsShArgs a) {
+  const int j = blockIdx.x, kk = blockIdx.y, t = threadIdx.x;
+  const int nmax = a.nmax, nf = nmax * (nmax + 2), S = nmax + 1;
+  const size_t sz = (size_t)nf * nf * nf;
+  constexpr int NQ = 4, CAP = kMaxNmax + 2;
+  __shared__ cplx jb[NQ][CAP], djb[NQ][CAP], d2jb[NQ][CAP], jsh[NQ][CAP], djsh[NQ][CAP];
+  __shared__ cplx cs[kMaxNmax * (kMaxNmax + 2)], ds[kMaxNmax * (kMaxNmax + 2)];
+  __shared__ cplx red[2 * NQ][256];
+  const double xi[NQ] = {-0.3399810435848563, 0.3399810435848563, -0.8611363115940526, 0.8611363115940526};
+  const double wi[NQ] = {0.6521451548625461, 0.6521451548625461, 0.3478548451374538, 0.3478548451374538};
+  const double R = a.radius[j];
+  const cplx kj = a.omega * csqrt_(a.eps[j] * a.mu[j]);
+  const cplx ksh = (2.0 * a.omega) * csqrt_(a.eps_sh[j] * a.mu[j]);
+  if(t < NQ) {
+    const double r = (R / 2.0) * xi[t] + R / 2.0;
+    cplx f[CAP], d[CAP];
+    const cplx z = r * kj;
+    sph_bessel(0, z, nmax, f, d);
+    for(int n = 0; n < S; ++n) {
+      jb[t][n] = f[n];
+      djb[t][n] = d[n];
+      cplx v = mk(0.0);
+      if(n >= 1 && hypot(z.x, z.y) > kErrEps) {
+        v = -(mk((double)n) / z) * d[n] + (mk((double)n) / (z * z)) * f[n] + d[n - 1];
+        if(z.y == 0.0) v.y = 0.0;
+      }
+      d2jb[t][n] = v;
+    }
+    sph_bessel(0, r * ksh, nmax, f, d);
+    for(int n = 0; n < S; ++n) {
+      jsh[t][n] = f[n];
+      djsh[t][n] = d[n];
+    }
+  }
+  for(int p = t; p < nf; p += blockDim.x) {
+    cs[p] = a.xint[(size_t)j * 2 * nf + p];
+    ds[p] = a.xint[(size_t)j * 2 * nf + nf + p];
+  }
+  __syncthreads();
+  int n, m;
+  unflat(kk, n, m);
+  const double snn = sqrt((double)n * (n + 1));
+  const cplx pre = (-mk(kEps0) / a.eps_sh[j]) * a.gamma[j];
+  const cplx ik2 = mk(1.0) / (kj * kj);
+  cplx xm1[NQ], xp1[NQ];
+  for(int i = 0; i < NQ; ++i) xm1[i] = xp1[i] = mk(0.0);
+  const double *W00 = a.tab + 6 * sz, *W11 = a.tab + 5 * sz, *Wm = a.tab + 4 * sz;
+  const size_t base = (size_t)kk * nf * nf;
+  for(int e = t; e < nf * nf; e += blockDim.x) {
+    const int p = e / nf, q = e % nf;
+    int n1, m1, n2, m2;
+    unflat(p, n1, m1);
+    unflat(q, n2, m2);
+    const double w00 = W00[base + e], w11 = W11[base + e], wm = Wm[base + e];
+    const double s4 = sqrt((double)n1 * n2 * (n1 + 1) * (n2 + 1));
+    const cplx cc = cs[p] * cs[q], dd = ds[p] * ds[q];
+#pragma unroll
+    for(int i = 0; i < NQ; ++i) {
+      const double r = (R / 2.0) * xi[i] + R / 2.0, ir = 1.0 / r;
+      const cplx j1 = jb[i][n1], j2 = jb[i][n2], dj1 = djb[i][n1], dj2 = djb[i][n2];
+      // F_d00, F_d11, F_dm1m1, F_00, F_11, F_m1m1 (Symbol.cpp:80-141)
+      const cplx fd00 = j1 * dj2 * kj + kj * dj1 * j2;
+      const cplx b1 = kj * dj1 + j1 * ir, b2 = kj * dj2 + j2 * ir;
+      const cplx a1 = kj * kj * d2jb[i][n1] - (ir * ir) * j1 + ir * kj * dj1;
+      const cplx a2 = kj * kj * d2jb[i][n2] - (ir * ir) * j2 + ir * kj * dj2;
+      const cplx fd11 = a1 * b2 + a2 * b1;
+      const cplx fdm = (ir * ir) * (kj * dj1 * j2 + j1 * kj * dj2) - (2.0 * ir * ir * ir) * (j1 * j2);
+      const cplx f00 = j1 * dj2, f11 = b2 * b1, fm = (ir * ir) * (j1 * j2);
+      xm1[i] = xm1[i] + pre * (cc * w00 * fd00 + dd * ik2 * (w11 * fd11 + (wm * s4) * fdm));
+      xp1[i] = xp1[i] + pre * (cc * (w00 * snn * ir) * f00 + dd * ik2 * ((w11 * snn * ir) * f11 + (wm * s4 * snn * ir) * fm));
+    }
+  }
+#pragma unroll
+  for(int i = 0; i < NQ; ++i) {
+    red[i][t] = xm1[i];
+    red[NQ + i][t] = xp1[i];
+  }
+  __syncthreads();
+  for(int st = 128; st > 0; st >>= 1) {
+    if(t < st)
+      for(int e = 0; e < 2 * NQ; ++e) red[e][t] = red[e][t] + red[e][t + st];
+    __syncthreads();
+  }
+  if(t == 0) {
+    const cplx k01 = mk(a.omega * sqrt(kEps0 * kMu0));
+    const cplx csh = a.xintsh[(size_t)j * 2 * nf + kk], dsh = a.xintsh[(size_t)j * 2 * nf + nf + kk];
+    cplx integral = mk(0.0);
+    for(int i = 0; i < NQ; ++i) {
+      const double r = (R / 2.0) * xi[i] + R / 2.0;
+      const cplx xm1sh = k01 * dsh * (mk(1.0) / ksh) * snn * (1.0 / r) * jsh[i][n];
+      const cplx x0 = -(k01 * csh * jsh[i][n]);
+      const cplx xp1sh = k01 * dsh * (mk(1.0) / ksh) * ((1.0 / r) * jsh[i][n] + djsh[i][n]);
+      const cplx em = red[i][0] + xm1sh, ep = red[NQ + i][0] + xp1sh;
+      integral = integral + (wi[i] * (r * r)) * (em * conj(em) + x0 * conj(x0) + ep * conj(ep));
+    }
+    a.coef[(size_t)j * nf + kk] = integral * (R / 2.0);
+  }
+}
+
+// Per object: Re((2 eta) 0.5 sigma_j sum_kk coefABS), harmonics summed in order.
+__global__ void abssh_sum_kernel(AbsShArgs a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if(j >= a.P) return;
+  const int nf = a.nmax * (a.nmax + 2);
+  cplx s = mk(0.0);
+  for(int kk = 0; kk < nf; ++kk) s = s + a.coef[(size_t)j * nf + kk];
+  const cplx eta = csqrt_(a.bg_mu / a.bg_eps);
+  const cplx eps_r_sh = a.eps_sh[j] / mk(kEps0);
+  const cplx sigma = -(mk(0.0, 1.0) * mk(kEps0) * (2.0 * a.omega)) * (eps_r_sh - mk(1.0));
+  a.out[j] = ((mk(2.0) * eta) * mk(0.5) * sigma * s).x;
+}
+
 void upload_factorials() {
   double h[171];
   h[0] = 1.0;
@@ -204,6 +329,12 @@ void run_cg(int nmax, double *out, hipStream_t st) {
   int nf = nmax * (nmax + 2);
   size_t sz = (size_t)nf * nf * nf;
   hipLaunchKernelGGL(cg_kernel, dim3((unsigned)((sz + 255) / 256)), dim3(256), 0, st, nf, out);
+}
+
+void run_abssh(const AbsShArgs &a, hipStream_t st) {
+  int nf = a.nmax * (a.nmax + 2);
+  hipLaunchKernelGGL(abssh_kernel, dim3(a.P, nf), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(abssh_sum_kernel, dim3((a.P + 63) / 64), dim3(64), 0, st, a);
 }
 
 void run_shsrc(const ShArgs &a, hipStream_t st) {

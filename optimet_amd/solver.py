@@ -28,7 +28,7 @@ EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "o
            "omx_set_solver", "omx_set_cg_tables", "omx_build_cg_tables", "omx_update", "omx_solve",
            "omx_cross_sections", "omx_scattering_size", "omx_get_stats", "omx_read_matrix", "omx_read_vector",
            "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_set_lu32_gemm", "omx_solve_sweep", "omx_mem_info", "omx_trim",
-           "omx_dense_solve_noalloc", "omx_read_matrix_block"]
+           "omx_dense_solve_noalloc", "omx_read_matrix_block", "omx_absorption_cross_section_sh"]
 
 
 class OmxError(RuntimeError):
@@ -90,6 +90,7 @@ def load_library(path=LIB_PATH):
     L.omx_trim.argtypes = [P]
     L.omx_dense_solve_noalloc.argtypes = [P, I, V, ctypes.c_size_t, V, V]
     L.omx_read_matrix_block.argtypes = [P, I, I, I, I, I, D]
+    L.omx_absorption_cross_section_sh.argtypes = [P, D]
     _lib = L
     return L
 
@@ -234,8 +235,8 @@ class Solver:
     def sweep(self, wavelengths, batch=0, outputs=("x_sca", "x_int", "x_sca_sh", "x_int_sh")):
         """Batched wavelength sweep (omx_solve_sweep): the cluster at every
         wavelength, Drude spheres re-evaluated per wavelength.  Returns a dict
-        with the requested (n_lambda, N) coefficient arrays, "xs" (n_lambda, 4:
-        C_ext, C_sca, C_abs, C_sca_SH) and "iterations" (n_lambda, 2)."""
+        with the requested (n_lambda, N) coefficient arrays, "xs" (n_lambda, 5:
+        C_ext, C_sca, C_abs, C_sca_SH, C_abs_SH) and "iterations" (n_lambda, 2)."""
         c = self.cluster
         lams = np.ascontiguousarray(np.asarray(wavelengths, dtype=np.float64))
         nl, N = lams.size, c.N
@@ -252,7 +253,7 @@ class Solver:
                 ptrs.append(_d(out[name].view(np.float64)))
             else:
                 ptrs.append(None)
-        xs = np.zeros((nl, 4))
+        xs = np.zeros((nl, 5))
         its = np.zeros((nl, 2), np.int32)
         _check(self.ctx.L.omx_solve_sweep(self.ctx.h, nl, _d(lams), _d(eps.view(np.float64)),
                                           _d(eps_sh.view(np.float64)), int(batch), *ptrs, _d(xs),
@@ -262,9 +263,12 @@ class Solver:
         return out
 
     def cross_sections(self):
-        cs = np.zeros(4)
+        """C_ext, C_sca, C_abs, C_sca_SH and, with the second harmonic, C_abs_SH."""
+        cs = np.zeros(5)
         _check(self.ctx.L.omx_cross_sections(self.ctx.h, _d(cs)))
-        return {"cext": cs[0], "csca": cs[1], "cabs": cs[2], "csca_sh": cs[3]}
+        if self.cluster.sh:
+            _check(self.ctx.L.omx_absorption_cross_section_sh(self.ctx.h, _d(cs[4:])))
+        return {"cext": cs[0], "csca": cs[1], "cabs": cs[2], "csca_sh": cs[3], "cabs_sh": cs[4]}
 
     def matrix(self, sh=False):
         N = self.cluster.N

@@ -83,6 +83,7 @@ def _declare(L):
     L.ora_source_sh.argtypes = [_P, _D, _D, _D, _D]
     L.ora_solve_all.argtypes = [_P, _I, _D, _D, _D, _D, _D]
     L.ora_cross_sections.argtypes = [_P, _D, _D, _D]
+    L.ora_absorption_sh.argtypes = [_P, _D, _D, _D, _D]
     L.ora_time_sample.argtypes = [_P, _I, _I, _I, _D]
 
 
@@ -287,13 +288,13 @@ def solve_all(prob, method=0):
     """Full update()+solve(): dict with x_sca, x_int, x_sca_sh, x_int_sh, xs."""
     N = prob.N
     xs_, xi_, xss, xis = _cz(N), _cz(N), _cz(N), _cz(N)
-    cs = np.zeros(4)
+    cs = np.zeros(5)
     p = _Pinned(prob)
     _check(lib().ora_solve_all(p.ref(), method, _ptr(xs_.view(np.float64)), _ptr(xi_.view(np.float64)),
                                _ptr(xss.view(np.float64)), _ptr(xis.view(np.float64)), _ptr(cs)))
     out = {"x_sca": xs_, "x_int": xi_, "cext": cs[0], "csca": cs[1], "cabs": cs[2]}
     if prob.sh:
-        out.update({"x_sca_sh": xss, "x_int_sh": xis, "csca_sh": cs[3]})
+        out.update({"x_sca_sh": xss, "x_int_sh": xis, "csca_sh": cs[3], "cabs_sh": cs[4]})
     return out
 
 
@@ -308,6 +309,17 @@ def cross_sections(prob, x_sca, x_sca_sh=None):
         _check(lib().ora_cross_sections(p.ref(), _ptr(xs.view(np.float64)),
                                         _ptr(xh.view(np.float64)), _ptr(cs)))
     return cs
+
+
+def absorption_sh(prob, tables, x_int, x_int_sh):
+    """Result::getAbsorptionCrossSection_SH from x_int and x_int_sh."""
+    t = np.ascontiguousarray(tables, dtype=np.float64).reshape(-1)
+    a = np.ascontiguousarray(x_int, dtype=np.complex128)
+    b = np.ascontiguousarray(x_int_sh, dtype=np.complex128)
+    out = np.zeros(1)
+    p = _Pinned(prob)
+    _check(lib().ora_absorption_sh(p.ref(), _ptr(t), _ptr(a.view(np.float64)), _ptr(b.view(np.float64)), _ptr(out)))
+    return float(out[0])
 
 
 def time_sample(prob, npairs, nsolve, method=1):

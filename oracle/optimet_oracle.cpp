@@ -986,6 +986,105 @@ void source_sh(const Problem &pb, const double *tab, const std::vector<cd> &x_in
   });
 }
 
+// ---------------------------------------------------------------- SH absorption
+// bessel3der (Bessel.h:147-245): the second-derivative array it returns,
+// d2[i] = -(i/z) j_i' + (i/z^2) j_i + j_{i-1}' for i >= 1, d2[0] = 0, kept
+// literally (the exact second derivative has i + 1 in place of i), real part
+// only on the real axis, all zero for |z| <= 1e-10.
+void bessel_d2(cd z, int nmax, std::vector<cd> &d, std::vector<cd> &dd, std::vector<cd> &d2) {
+  bessel(kBessel, z, nmax, d, dd);
+  d2.assign(nmax + 1, cd(0, 0));
+  if(std::abs(z) <= kErrEps) return;
+  for(int i = 1; i <= nmax; ++i) {
+    d2[i] = -((double)i / z) * dd[i] + ((double)i / (z * z)) * d[i] + dd[i - 1];
+    if(z.imag() == 0.0) d2[i] = d2[i].real();
+  }
+}
+
+// Geometry::AbsCSSHcoeff -> symbol::ACSshcoeff (Geometry.cpp:428-455,
+// Symbol.cpp:358-470) for every output harmonic kk of object obj: the
+// 4-point Gauss-Legendre radial integral over [0, R] of |E(2 omega)|^2 built
+// from the bulk (gamma) source of the fundamental internal field and the
+// internal SH field; summed over kk.  x_int and x_int_sh enter unconjugated.
+// The SH term of the X(+1) component adds j_n'(K r) without its factor K
+// (Symbol.cpp:460), kept literally.
+cd abs_sh_object(const Problem &pb, const double *tab, const cd *x_int, const cd *x_int_sh, int obj) {
+  const int nmax = pb.nmax, nf = pb.nf;
+  const size_t sz = (size_t)nf * nf * nf;
+  const double *Wm1m1 = tab + 4 * sz, *W11 = tab + 5 * sz, *W00 = tab + 6 * sz;
+  const Object &o = pb.obj[obj];
+  const double R = o.radius, omega = pb.omega;
+  const cd eps_0 = kEps0, mu_0 = kMu0;
+  const cd k01 = omega * std::sqrt(eps_0 * mu_0), kj = omega * std::sqrt(o.eps * o.mu);
+  const cd ksh = 2.0 * omega * std::sqrt(o.eps_sh * o.mu_sh);
+  const cd pre = (-eps_0 / o.eps_sh) * o.gamma;
+  // Gauss-Legendre nodes and weights on [-1, 1] (Symbol.cpp:397-398)
+  const double xi[4] = {-0.3399810435848563, 0.3399810435848563, -0.8611363115940526, 0.8611363115940526};
+  const double wi[4] = {0.6521451548625461, 0.6521451548625461, 0.3478548451374538, 0.3478548451374538};
+  const cd *c = x_int + (size_t)obj * 2 * nf, *d = c + nf;
+  const cd *csh = x_int_sh + (size_t)obj * 2 * nf, *dsh = csh + nf;
+  std::vector<int> nof(nf);
+  for(int p = 0; p < nf; ++p) nof[p] = (int)std::sqrt(p + 1.0);
+  cd total = 0.0;
+  for(int kk = 0; kk < nf; ++kk) {
+    const int n = nof[kk];
+    const double snn = std::sqrt((double)n * (n + 1));
+    cd integral = 0.0;
+    for(int ii = 0; ii < 4; ++ii) {
+      const double r = (R / 2.0) * xi[ii] + R / 2.0;
+      std::vector<cd> j, dj, d2j, jsh, djsh;
+      bessel_d2(r * kj, nmax, j, dj, d2j);
+      bessel(kBessel, r * ksh, nmax, jsh, djsh);
+      cd xm1 = 0.0, xp1 = 0.0;
+      for(int p = 0; p < nf; ++p) {
+        const int n1 = nof[p];
+        for(int q = 0; q < nf; ++q) {
+          const int n2 = nof[q];
+          const size_t t = (size_t)kk * nf * nf + (size_t)p * nf + q;
+          const double w00 = W00[t], w11 = W11[t], wm = Wm1m1[t];
+          const double s4 = std::sqrt((double)n1 * n2 * (n1 + 1) * (n2 + 1));
+          const cd cc = c[p] * c[q], ddp = d[p] * d[q];
+          // F_d00, F_d11, F_dm1m1, F_00, F_11, F_m1m1 (Symbol.cpp:80-141)
+          const cd fd00 = j[n1] * dj[n2] * kj + kj * dj[n1] * j[n2];
+          const cd a1 = kj * kj * d2j[n1] - (1.0 / (r * r)) * j[n1] + (1.0 / r) * kj * dj[n1];
+          const cd a2 = kj * kj * d2j[n2] - (1.0 / (r * r)) * j[n2] + (1.0 / r) * kj * dj[n2];
+          const cd b1 = kj * dj[n1] + j[n1] / r, b2 = kj * dj[n2] + j[n2] / r;
+          const cd fd11 = a1 * b2 + a2 * b1;
+          const cd fdm1m1 = (1.0 / (r * r)) * (kj * dj[n1] * j[n2] + j[n1] * kj * dj[n2]) - (2.0 / (r * r * r)) * (j[n1] * j[n2]);
+          const cd f00 = j[n1] * dj[n2];
+          const cd f11 = b2 * b1;
+          const cd fm1m1 = (1.0 / (r * r)) * (j[n1] * j[n2]);
+          xm1 += pre * (cc * w00 * fd00 + ddp * (1.0 / (kj * kj)) * (w11 * fd11 + wm * s4 * fdm1m1));
+          xp1 += pre * (cc * w00 * snn * (1.0 / r) * f00 +
+                        ddp * (1.0 / (kj * kj)) * (w11 * snn * (1.0 / r) * f11 + wm * s4 * snn * (1.0 / r) * fm1m1));
+        }
+      }
+      const cd xm1sh = k01 * dsh[kk] * (1.0 / ksh) * snn * (1.0 / r) * jsh[n];
+      const cd x0 = -k01 * csh[kk] * jsh[n];
+      const cd xp1sh = k01 * dsh[kk] * (1.0 / ksh) * ((1.0 / r) * jsh[n] + djsh[n]);
+      const cd em = xm1 + xm1sh, ep = xp1 + xp1sh;
+      integral += wi[ii] * (r * r) * (em * std::conj(em) + x0 * std::conj(x0) + ep * std::conj(ep));
+    }
+    total += integral * (R / 2.0);
+  }
+  return total;
+}
+
+// Result::getAbsorptionCrossSection_SH (Result.cpp:763-794):
+// sum over objects of Re(2 eta 0.5 sigma sum_kk coefABS), sigma = -i eps0 2 omega (eps_r_SH - 1),
+// eta = sqrt(mu_b / eps_b).
+double abs_sh(const Problem &pb, const double *tab, const cd *x_int, const cd *x_int_sh) {
+  const cd eta = std::sqrt(pb.bg_mu / pb.bg_eps);
+  std::vector<double> part(pb.P, 0.0);
+  parallel_for(pb.P, [&](int j) {
+    const cd sigma = -cd(0.0, 1.0) * kEps0 * 2.0 * pb.omega * (pb.obj[j].eps_r_sh - 1.0);
+    part[j] = std::real((2.0 * eta) * 0.5 * sigma * abs_sh_object(pb, tab, x_int, x_int_sh, j));
+  });
+  double s = 0.0;
+  for(double v : part) s += v;
+  return s;
+}
+
 // ---------------------------------------------------------------- cross sections
 // Geometry::getCabsAux (Geometry.cpp:86-145)
 void cabs_aux(const Problem &pb, const Object &o, std::vector<double> &out) {
@@ -1260,7 +1359,15 @@ int ora_solve_all(const ora_problem *in, int method, double *x_sca, double *x_in
       std::memcpy(x_sca_sh, XS.data(), N * sizeof(cd));
       std::memcpy(x_int_sh, XIS.data(), N * sizeof(cd));
     }
-    if(xs) cross_sections(pb, X.data(), pb.sh ? XS.data() : nullptr, xs);
+    if(xs) {
+      cross_sections(pb, X.data(), pb.sh ? XS.data() : nullptr, xs);
+      xs[4] = 0.0;
+      if(pb.sh) {
+        std::vector<double> tab((size_t)9 * nf * nf * nf);
+        cg_tables(pb.nmax, pb.nmax, tab.data());
+        xs[4] = abs_sh(pb, tab.data(), XI.data(), XIS.data());
+      }
+    }
   });
 }
 
@@ -1312,6 +1419,14 @@ int ora_time_sample(const ora_problem *in, int npairs, int nsolve, int method, d
     t0 = now();
     inc_local_sh(pb, tab.data(), u, 0, ana.data());
     out[2] = std::chrono::duration<double>(now() - t0).count();
+  });
+}
+
+int ora_absorption_sh(const ora_problem *in, const double *tables, const double *x_int, const double *x_int_sh,
+                      double *out) {
+  return guard([&] {
+    Problem pb = make_problem(in);
+    *out = abs_sh(pb, tables, (const cd *)x_int, (const cd *)x_int_sh);
   });
 }
 

@@ -106,13 +106,18 @@ int ora_source_sh(const ora_problem *pb, const double *tables, const double *x_i
                   double *K1ana);
 
 /* Full PreconditionedMatrix update()+solve(): x_sca, x_int (N complex each) and,
- * if pb->sh, x_sca_sh, x_int_sh.  xs: [Cext, Csca, Cabs, Csca_SH]. */
+ * if pb->sh, x_sca_sh, x_int_sh.  xs: [Cext, Csca, Cabs, Csca_SH, Cabs_SH]. */
 int ora_solve_all(const ora_problem *pb, int method, double *x_sca, double *x_int,
                   double *x_sca_sh, double *x_int_sh, double *xs);
 
 /* Cross sections from solved coefficients: xs = [Cext, Csca, Cabs, Csca_SH]. */
 int ora_cross_sections(const ora_problem *pb, const double *x_sca, const double *x_sca_sh,
                        double *xs);
+
+/* Result::getAbsorptionCrossSection_SH from the internal coefficients at omega and 2 omega
+ * (N complex each) and the nine tables of ora_cg_tables. */
+int ora_absorption_sh(const ora_problem *pb, const double *tables, const double *x_int,
+                      const double *x_int_sh, double *out);
 
 /* CPU-baseline timing sample: out[0] s per pair coupling + block, out[1] s
  * of one dense solve of order nsolve (method as ora_dense_solve), out[2] s of

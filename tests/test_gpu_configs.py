@@ -144,6 +144,9 @@ def test_config2_full_size(ora, config2_ref, kind):
         cs = s.cross_sections()
         check_solution(f"config2 {kind}", out, cs, ref, TOL[kind], ora, c)
         assert rel(out["x_int"], out["x_sca"] * iaux(ora, c)) < 1e-12
+        # C_abs,SH of the device's own internal coefficients (Result.cpp:763-794)
+        cabs_sh = ora.absorption_sh(c, ora_tables(ora, c.nmax), out["x_int"], out["x_int_sh"])
+        assert relabs(cs["cabs_sh"], cabs_sh) < 1e-12, relabs(cs["cabs_sh"], cabs_sh)
         # the second-harmonic source of the device's own x_int (Geometry.cpp:250-310)
         K, _ = ora.source_sh(c, ora_tables(ora, c.nmax), out["x_int"])
         assert rel(s.vector("K"), K) < 1e-11, rel(s.vector("K"), K)

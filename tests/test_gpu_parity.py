@@ -115,6 +115,10 @@ def test_second_harmonic_chain(ctx, ora, kind, tol):
     assert rel(out["x_int_sh"], ref["x_int_sh"]) < tol
     cs = s.cross_sections()
     assert abs(cs["csca_sh"] - ref["csca_sh"]) <= tol * abs(ref["csca_sh"])
+    assert abs(cs["cabs_sh"] - ref["cabs_sh"]) <= tol * abs(ref["cabs_sh"])
+    # C_abs,SH evaluated from the device's own internal coefficients
+    own = ora.absorption_sh(c, ora.cg_tables(c.nmax), out["x_int"], out["x_int_sh"])
+    assert abs(cs["cabs_sh"] - own) <= 1e-12 * abs(own)
 
 
 @pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL)])
@@ -131,7 +135,7 @@ def test_nmax9_cluster(ctx, ora, kind, tol):
     cs = s.cross_sections()
     assert rel(out["x_sca"], ref["x_sca"]) < tol
     assert rel(out["x_sca_sh"], ref["x_sca_sh"]) < tol
-    for key in ("cext", "csca", "cabs", "csca_sh"):
+    for key in ("cext", "csca", "cabs", "csca_sh", "cabs_sh"):
         assert abs(cs[key] - ref[key]) <= tol * abs(ref[key]), key
 
 
@@ -337,7 +341,7 @@ def test_wavelength_sweep_batched(ctx, ora, kind, tol, P, nmax, batch):
         assert rel(out["x_sca"][i], ref["x_sca"]) < tol, (i, lam)
         assert rel(out["x_sca_sh"][i], ref["x_sca_sh"]) < tol, (i, lam)
         xs = out["xs"][i]
-        for j, key in enumerate(("cext", "csca", "cabs", "csca_sh")):
+        for j, key in enumerate(("cext", "csca", "cabs", "csca_sh", "cabs_sh")):
             assert abs(xs[j] - ref[key]) <= tol * abs(ref[key]), (i, key)
         assert out["iterations"][i, 0] > 0
 

@@ -565,3 +565,22 @@ def test_sh_cross_section_scaling_laws(ora):
     assert abs(r3["csca_sh"] / ref["csca_sh"] - 16.0) < 1e-11
     assert abs(r2["cext"] / ref["cext"] - 1.0) < 1e-14
     assert np.linalg.norm(r3["x_sca"] - 2 * ref["x_sca"]) < 1e-13 * np.linalg.norm(ref["x_sca"])
+
+
+def test_sh_absorption_signs_and_scaling(ora):
+    """C_abs,SH (Result.cpp:763-794) = sum_j Re(eta sigma_j I_j), sigma_j =
+    -i eps0 2 omega (eps_r_SH - 1), I_j the radial integral of |E(2 omega)|^2
+    (real, positive): it is positive for an absorbing SH permittivity, zero
+    for a lossless one, quadratic in the susceptibilities and quartic in the
+    pump amplitude."""
+    import dataclasses
+    from optimet_amd.problem import drude_cluster
+    c = drude_cluster(3, 3, 700e-9, 40, 80, 500, 10, seed=3)
+    ref = ora.solve_all(c)
+    assert ref["cabs_sh"] > 0 and c.eps_r_sh[0].imag > 0
+    lossless = dataclasses.replace(c, eps_r_sh=np.full(c.nobj, 2.25 + 0j))
+    assert abs(ora.solve_all(lossless)["cabs_sh"]) < 1e-13 * ref["cabs_sh"]
+    c2 = dataclasses.replace(c, ksippp=2 * c.ksippp, ksiparppar=2 * c.ksiparppar, gamma=2 * c.gamma)
+    c3 = dataclasses.replace(c, etheta=2 * c.etheta)
+    assert abs(ora.solve_all(c2)["cabs_sh"] / ref["cabs_sh"] - 4.0) < 1e-12
+    assert abs(ora.solve_all(c3)["cabs_sh"] / ref["cabs_sh"] - 16.0) < 1e-11
