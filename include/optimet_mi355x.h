@@ -92,6 +92,7 @@ typedef struct {
   int sweep_batch;       /* systems per batch of the last omx_solve_sweep */
   int matvec_calls;      /* FP64 matrix-vector products of the iterative solvers (since omx_set_profile) */
   double t_matvec_ms;    /* their HIP-event time (profiling on) */
+  double matvec_bytes;   /* their algorithmic bytes, 16 Np^2 per system and product */
 } omx_stats;
 
 const char *omx_last_error(void);

@@ -47,6 +47,14 @@ __device__ inline void riccati_set(cplx r0, cplx rho, int nmax, cplx *psi, cplx 
 __global__ void objects_kernel(ObjArgs a) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= a.P) return;
+  if(a.omz) {  // system blockIdx.z of a batch
+    const int z = blockIdx.z;
+    a.omega = a.omz[z];
+    a.eps += (size_t)z * a.P;
+    a.eps_sh += (size_t)z * a.P;
+    a.ov += z * a.zov;
+    if(z > 0) a.sph = nullptr;  // the geometry is common: system 0 writes it
+  }
   const int nmax = a.nmax, S = nmax + 1;
   auto OV = [&](int kind, int pol, int n) -> cplx & {
     return a.ov[(((size_t)kind * a.P + i) * 2 + pol) * S + n];
@@ -54,12 +62,14 @@ __global__ void objects_kernel(ObjArgs a) {
   // spherical coordinates of the centre (Tools.cpp:250-258)
   double x = a.xyz[3 * i], y = a.xyz[3 * i + 1], z = a.xyz[3 * i + 2];
   double r = sqrt(x * x + y * y + z * z);
-  if(r > 0.0) {
-    a.sph[3 * i] = r;
-    a.sph[3 * i + 1] = acos(z / r);
-    a.sph[3 * i + 2] = atan2(y, x);
-  } else {
-    a.sph[3 * i] = a.sph[3 * i + 1] = a.sph[3 * i + 2] = 0.0;
+  if(a.sph) {
+    if(r > 0.0) {
+      a.sph[3 * i] = r;
+      a.sph[3 * i + 1] = acos(z / r);
+      a.sph[3 * i + 2] = atan2(y, x);
+    } else {
+      a.sph[3 * i] = a.sph[3 * i + 1] = a.sph[3 * i + 2] = 0.0;
+    }
   }
   const double R = a.radius[i];
   const cplx eps = a.eps[i], mu = a.mu[i], eps_sh = a.eps_sh[i], mu_sh = mu;
@@ -320,6 +330,13 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs a) {
   constexpr int NF = PairLDS<NMAX>::NF;
   __shared__ PairLDS<NMAX> s;
   const int P = a.P;
+  if(a.kz) {  // system blockIdx.z of a batch
+    const int z = blockIdx.z;
+    a.k = a.kz[z];
+    a.ov += z * a.zov;
+    if(a.S64) a.S64 += z * a.zS;
+    if(a.S32) a.S32 += z * a.zS;
+  }
   const int pid = blockIdx.x;
   const int i = pid / P, j = pid % P;
   const size_t x0 = (size_t)i * 2 * NF, y0 = (size_t)j * 2 * NF;
@@ -394,6 +411,14 @@ __global__ __launch_bounds__(256) void origin_kernel(OriginArgs a) {
   const int j = blockIdx.x, fq = blockIdx.y;
   const int t = threadIdx.x;
   const int P = a.P;
+  if(a.kz) {  // system blockIdx.z of a batch
+    const int z = blockIdx.z;
+    a.k = a.kz[z];
+    a.ov += z * a.zov;
+    a.incloc += z * a.zvec;
+    a.Q += z * a.zvec;
+    a.w += z * a.zw;
+  }
   // incident vector [a_inc; b_inc] in s.Ti
   for(int r = t; r < 2 * NF; r += blockDim.x) s.Ti[r] = a.inc[r];
   for(int r = t; r < NF; r += blockDim.x) s.w[r] = 0.0;
@@ -488,7 +513,8 @@ __global__ void cross_kernel(CrossArgs a) {
 // Rows/columns N..Np-1 of an Np x Np matrix: identity on the diagonal, zero
 // elsewhere, so the padded system has the same solution in its first N rows.
 template <typename T>
-__global__ void pad_kernel(T *A, int N, int Np, size_t lda) {
+__global__ void pad_kernel(T *A, int N, int Np, size_t lda, size_t zs) {
+  A += blockIdx.z * zs;
   size_t c = blockIdx.y;
   for(int r = blockIdx.x * blockDim.x + threadIdx.x; r < Np; r += gridDim.x * blockDim.x) {
     bool pad = (c >= (size_t)N) || (r >= N);
@@ -501,12 +527,12 @@ __global__ void pad_kernel(T *A, int N, int Np, size_t lda) {
 
 // ------------------------------------------------------------------ launchers
 template <int NMAX>
-static void launch_pair(const PairArgs &a, hipStream_t st) {
-  hipLaunchKernelGGL(pair_kernel<NMAX>, dim3(a.P * a.P), dim3(256), 0, st, a);
+static void launch_pair(const PairArgs &a, hipStream_t st, int nb) {
+  hipLaunchKernelGGL(pair_kernel<NMAX>, dim3(a.P * a.P, 1, nb), dim3(256), 0, st, a);
 }
 template <int NMAX>
-static void launch_origin(const OriginArgs &a, hipStream_t st) {
-  hipLaunchKernelGGL(origin_kernel<NMAX>, dim3(a.P, 2), dim3(256), 0, st, a);
+static void launch_origin(const OriginArgs &a, hipStream_t st, int nb) {
+  hipLaunchKernelGGL(origin_kernel<NMAX>, dim3(a.P, 2, nb), dim3(256), 0, st, a);
 }
 
 #define OMX_NMAX_SWITCH(nmax, F, ...) \
@@ -524,22 +550,22 @@ static void launch_origin(const OriginArgs &a, hipStream_t st) {
   default: break;                     \
   }
 
-void run_objects(const ObjArgs &a, hipStream_t st) {
-  hipLaunchKernelGGL(objects_kernel, dim3((a.P + 63) / 64), dim3(64), 0, st, a);
+void run_objects(const ObjArgs &a, hipStream_t st, int nb) {
+  hipLaunchKernelGGL(objects_kernel, dim3((a.P + 63) / 64, 1, nb), dim3(64), 0, st, a);
 }
-void run_pairs(const PairArgs &a, hipStream_t st) { OMX_NMAX_SWITCH(a.nmax, launch_pair, a, st); }
-void run_origin(const OriginArgs &a, hipStream_t st) { OMX_NMAX_SWITCH(a.nmax, launch_origin, a, st); }
+void run_pairs(const PairArgs &a, hipStream_t st, int nb) { OMX_NMAX_SWITCH(a.nmax, launch_pair, a, st, nb); }
+void run_origin(const OriginArgs &a, hipStream_t st, int nb) { OMX_NMAX_SWITCH(a.nmax, launch_origin, a, st, nb); }
 void run_cross(const CrossArgs &a, hipStream_t st) {
   hipLaunchKernelGGL(cross_kernel, dim3(a.P), dim3(256), 0, st, a);
 }
-void run_pad64(cplx *A, int N, int Np, size_t lda, hipStream_t st) {
+void run_pad64(cplx *A, int N, int Np, size_t lda, hipStream_t st, int nb, size_t zs) {
   if(Np == N) return;
-  hipLaunchKernelGGL(pad_kernel<double2>, dim3((Np + 255) / 256, Np), dim3(256), 0, st,
-                     reinterpret_cast<double2 *>(A), N, Np, lda);
+  hipLaunchKernelGGL(pad_kernel<double2>, dim3((Np + 255) / 256, Np, nb), dim3(256), 0, st,
+                     reinterpret_cast<double2 *>(A), N, Np, lda, zs);
 }
-void run_pad32(float2 *A, int N, int Np, size_t lda, hipStream_t st) {
+void run_pad32(float2 *A, int N, int Np, size_t lda, hipStream_t st, int nb, size_t zs) {
   if(Np == N) return;
-  hipLaunchKernelGGL(pad_kernel<float2>, dim3((Np + 255) / 256, Np), dim3(256), 0, st, A, N, Np, lda);
+  hipLaunchKernelGGL(pad_kernel<float2>, dim3((Np + 255) / 256, Np, nb), dim3(256), 0, st, A, N, Np, lda, zs);
 }
 
 // Coefficient table: a_plus, a_minus, b_plus, b_minus, 1/a_plus, 1/b_plus at

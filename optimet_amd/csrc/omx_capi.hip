@@ -131,12 +131,12 @@ struct omx_context {
     DBuf S64, S32, V64, V32, F64, F64sh, eps, eps_sh, ov, incloc, Q, w, x, xint, xsh, xintsh, K, K1, cross, abscoef,
         abssh;
     DBuf perm, perm2, flags, flags2, dinv, dinv2, map, status, basis, basisZ, r, w1, w2, gpart, part, hd, nrm, sc,
-        act;
+        act, kz, omz;
   } sw;
   omx_stats stats{};
   void *scratch = nullptr;  // this device's address of the no-allocation solve's scratch array
   long long mv_calls = 0;  // FP64 matrix-vector products since the last omx_set_profile
-  double mv_ms = 0;
+  double mv_ms = 0, mv_bytes = 0;
   double omega() const { return omx::kC0 * (2.0 * omx::kPi / lambda); }
   cplx waveK() const {
     std::complex<double> b = std::sqrt(std::complex<double>(bg_eps_r.x, bg_eps_r.y) *
@@ -301,6 +301,7 @@ struct Gmres {
     if(c->profile) HIPCHK(hipEventRecord(c->ev[6], s));
     omx::gemv64(A, (size_t)c->Np, N, x, y, c->d_gpart.as<cplx>(), s);
     ++c->mv_calls;
+    c->mv_bytes += 16.0 * c->Np * (double)c->Np;
     if(c->profile) {
       HIPCHK(hipEventRecord(c->ev[7], s));
       HIPCHK(hipEventSynchronize(c->ev[7]));
@@ -605,7 +606,19 @@ struct GmresB {
     for(double &e : v) e = std::sqrt(e);
     return v;
   }
-  void matvec(const cplx *x, cplx *y) { omx::gemv64(A, (size_t)Np, Np, x, y, w.gpart.as<cplx>(), c->st, nb, sA, sv()); }
+  void matvec(const cplx *x, cplx *y) {
+    if(c->profile) HIPCHK(hipEventRecord(c->ev[4], c->st));
+    omx::gemv64(A, (size_t)Np, Np, x, y, w.gpart.as<cplx>(), c->st, nb, sA, sv());
+    ++c->mv_calls;
+    c->mv_bytes += 16.0 * Np * (double)Np * nb;
+    if(c->profile) {
+      HIPCHK(hipEventRecord(c->ev[5], c->st));
+      HIPCHK(hipEventSynchronize(c->ev[5]));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev[4], c->ev[5]));
+      c->mv_ms += ms;
+    }
+  }
   // per-system scalars a[z], b[z] in device memory
   void scale(cplx *y, size_t sy, const std::vector<cplx> &a, const cplx *x, size_t sx, const std::vector<cplx> &b,
              const int *act) {
@@ -811,6 +824,8 @@ void sweep_alloc(omx_context *c, int nb, bool f64, bool sh) {
   w.nrm.ensure(sizeof(double) * 160 * nb, T);
   w.sc.ensure(sizeof(cplx) * 2 * nb, T);
   w.act.ensure(sizeof(int) * nb, T);
+  w.kz.ensure(sizeof(cplx) * 2 * nb, T);
+  w.omz.ensure(sizeof(double) * nb, T);
 }
 
 omx::LuWork sweep_lu(omx_context *c, int nb, bool second) {
@@ -878,32 +893,49 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
   omx::run_incident(ia, st);
   cplx *cS64 = w.S64.as<cplx>(), *cV64 = w.V64.as<cplx>();
   float2 *cS32 = w.S32.as<float2>(), *cV32 = w.V32.as<float2>();
+  // per-system wavenumbers (k at omega, 2k at 2 omega) and angular frequencies;
+  // every assembly kernel then covers the whole batch in one launch (system in blockIdx.z)
+  std::vector<cplx> hk(2 * nb);
+  std::vector<double> hom(nb);
   for(int z = 0; z < nb; ++z) {
-    const double k0 = 2.0 * omx::kPi / lam[z], omega = omx::kC0 * k0;
-    const cplx k = omx::mk(k0 * bgn.real(), k0 * bgn.imag());
-    const cplx *ez = w.eps.as<cplx>() + (size_t)z * P, *esz = w.eps_sh.as<cplx>() + (size_t)z * P;
-    cplx *ov = w.ov.as<cplx>() + z * ovs;
-    omx::ObjArgs oa{P, nmax, omega, bge, bgm, c->d_xyz.as<double>(), c->d_radius.as<double>(), ez,
-                    c->d_mu.as<cplx>(), esz, c->d_sph.as<double>(), ov};
-    omx::run_objects(oa, st);
-    omx::OriginArgs og{P, nmax, k, c->d_sph.as<double>(), ov, c->d_coef.as<double>(), c->d_inc.as<cplx>(),
-                       w.incloc.as<cplx>() + z * lda, w.Q.as<cplx>() + z * lda, w.w.as<double>() + z * ws};
-    omx::run_origin(og, st);
-    omx::PairArgs pa{P, nmax, omx::OV_T, k, c->d_sph.as<double>(), ov, c->d_coef.as<double>(), lda,
-                     cS64 + z * n2, f64 ? nullptr : cS32 + z * n2};
-    omx::run_pairs(pa, st);
-    omx::run_pad64(cS64 + z * n2, N, Np, lda, st);
-    if(!f64) omx::run_pad32(cS32 + z * n2, N, Np, lda, st);
-    if(sh) {
-      omx::PairArgs pv = pa;
-      pv.tkind = omx::OV_TSH;
-      pv.k = omx::mk(2.0 * k.x, 2.0 * k.y);  // PreconditionedMatrix.cpp:592
-      pv.S64 = cV64 + z * n2;
-      pv.S32 = f64 ? nullptr : cV32 + z * n2;
-      omx::run_pairs(pv, st);
-      omx::run_pad64(cV64 + z * n2, N, Np, lda, st);
-      if(!f64) omx::run_pad32(cV32 + z * n2, N, Np, lda, st);
-    }
+    const double k0 = 2.0 * omx::kPi / lam[z];
+    hom[z] = omx::kC0 * k0;
+    hk[z] = omx::mk(k0 * bgn.real(), k0 * bgn.imag());
+    hk[nb + z] = omx::mk(2.0 * hk[z].x, 2.0 * hk[z].y);  // PreconditionedMatrix.cpp:592
+  }
+  HIPCHK(hipMemcpyAsync(w.kz.p, hk.data(), sizeof(cplx) * 2 * nb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(w.omz.p, hom.data(), sizeof(double) * nb, hipMemcpyHostToDevice, st));
+  const cplx *kz = w.kz.as<cplx>();
+  omx::ObjArgs oa{P, nmax, hom[0], bge, bgm, c->d_xyz.as<double>(), c->d_radius.as<double>(), w.eps.as<cplx>(),
+                  c->d_mu.as<cplx>(), w.eps_sh.as<cplx>(), c->d_sph.as<double>(), w.ov.as<cplx>()};
+  oa.omz = w.omz.as<double>();
+  oa.zov = ovs;
+  omx::run_objects(oa, st, nb);
+  omx::OriginArgs og{P, nmax, hk[0], c->d_sph.as<double>(), w.ov.as<cplx>(), c->d_coef.as<double>(),
+                     c->d_inc.as<cplx>(), w.incloc.as<cplx>(), w.Q.as<cplx>(), w.w.as<double>()};
+  og.kz = kz;
+  og.zov = ovs;
+  og.zvec = lda;
+  og.zw = ws;
+  omx::run_origin(og, st, nb);
+  omx::PairArgs pa{P, nmax, omx::OV_T, hk[0], c->d_sph.as<double>(), w.ov.as<cplx>(), c->d_coef.as<double>(), lda,
+                   cS64, f64 ? nullptr : cS32};
+  pa.kz = kz;
+  pa.zov = ovs;
+  pa.zS = n2;
+  omx::run_pairs(pa, st, nb);
+  omx::run_pad64(cS64, N, Np, lda, st, nb, n2);
+  if(!f64) omx::run_pad32(cS32, N, Np, lda, st, nb, n2);
+  if(sh) {
+    omx::PairArgs pv = pa;
+    pv.tkind = omx::OV_TSH;
+    pv.k = hk[nb];
+    pv.kz = kz + nb;
+    pv.S64 = cV64;
+    pv.S32 = f64 ? nullptr : cV32;
+    omx::run_pairs(pv, st, nb);
+    omx::run_pad64(cV64, N, Np, lda, st, nb, n2);
+    if(!f64) omx::run_pad32(cV32, N, Np, lda, st, nb, n2);
   }
   HIPCHK(hipGetLastError());
   auto status_scan = [&](std::vector<int> &flag) {
@@ -1144,7 +1176,7 @@ void omx_destroy(omx_context *c) {
   DBuf *sbufs[] = {&w.S64, &w.S32, &w.V64, &w.V32, &w.F64, &w.F64sh, &w.eps, &w.eps_sh, &w.ov, &w.incloc, &w.Q,
                    &w.w, &w.x, &w.xint, &w.xsh, &w.xintsh, &w.K, &w.K1, &w.cross, &w.perm, &w.perm2, &w.flags,
                    &w.flags2, &w.dinv, &w.dinv2, &w.map, &w.status, &w.basis, &w.basisZ, &w.r, &w.w1, &w.w2, &w.gpart,
-                   &w.part, &w.hd, &w.nrm, &w.sc, &w.act, &w.abscoef, &w.abssh};
+                   &w.part, &w.hd, &w.nrm, &w.sc, &w.act, &w.abscoef, &w.abssh, &w.kz, &w.omz};
   for(DBuf *b : sbufs) b->release(&c->total);
   for(auto &e : c->ev)
     if(e) (void)hipEventDestroy(e);
@@ -1501,6 +1533,7 @@ int omx_get_stats(const omx_context *c, omx_stats *out) {
   out->pivot_fallbacks = c->fallbacks;
   out->matvec_calls = (int)c->mv_calls;
   out->t_matvec_ms = c->mv_ms;
+  out->matvec_bytes = c->mv_bytes;
   return OMX_OK;
 }
 
@@ -1512,6 +1545,7 @@ int omx_set_profile(omx_context *c, int on) {
     c->prof_launches = 0;
     c->mv_calls = 0;
     c->mv_ms = 0;
+    c->mv_bytes = 0;
   });
 }
 

@@ -31,6 +31,10 @@ struct ObjArgs {
   const cplx *eps, *mu, *eps_sh;     // absolute
   double *sph;                       // out: 3P spherical coordinates
   cplx *ov;                          // out: per-object vectors
+  // batches (blockIdx.z = system z): omega, eps, eps_sh and ov of system z at
+  // omz[z], eps + z P, eps_sh + z P, ov + z zov (omz null: one system)
+  const double *omz = nullptr;
+  size_t zov = 0;
 };
 
 struct IncArgs {
@@ -49,6 +53,9 @@ struct PairArgs {
   size_t lda;
   cplx *S64;                         // optional FP64 output
   float2 *S32;                       // optional complex-FP32 output
+  // batches (blockIdx.z = system z): wavenumber kz[z], ov + z zov, matrices + z zS
+  const cplx *kz = nullptr;
+  size_t zov = 0, zS = 0;
 };
 
 struct OriginArgs {
@@ -61,6 +68,10 @@ struct OriginArgs {
   cplx *incloc;                      // out: N
   cplx *Q;                           // out: N
   double *w;                         // out: [2][P][nf]
+  // batches (blockIdx.z = system z): wavenumber kz[z], ov + z zov,
+  // incloc/Q + z zvec, w + z zw
+  const cplx *kz = nullptr;
+  size_t zov = 0, zvec = 0, zw = 0;
 };
 
 struct CrossArgs {
@@ -71,13 +82,15 @@ struct CrossArgs {
   double *out;                       // [P][4]
 };
 
-void run_objects(const ObjArgs &a, hipStream_t st);
+// nb > 1: a batch of nb systems in one launch (the batch fields of the args)
+void run_objects(const ObjArgs &a, hipStream_t st, int nb = 1);
 void run_incident(const IncArgs &a, hipStream_t st);
-void run_pairs(const PairArgs &a, hipStream_t st);
-void run_origin(const OriginArgs &a, hipStream_t st);
+void run_pairs(const PairArgs &a, hipStream_t st, int nb = 1);
+void run_origin(const OriginArgs &a, hipStream_t st, int nb = 1);
 void run_cross(const CrossArgs &a, hipStream_t st);
-void run_pad64(cplx *A, int N, int Np, size_t lda, hipStream_t st);
-void run_pad32(float2 *A, int N, int Np, size_t lda, hipStream_t st);
+// identity padding of nb matrices zs elements apart
+void run_pad64(cplx *A, int N, int Np, size_t lda, hipStream_t st, int nb = 1, size_t zs = 0);
+void run_pad32(float2 *A, int N, int Np, size_t lda, hipStream_t st, int nb = 1, size_t zs = 0);
 void fill_coef(double *host);
 
 // ---------------------------------------------------------------- LU

@@ -155,8 +155,17 @@ __global__ __launch_bounds__(256) void shsrc_kernel(ShArgs a) {
   __syncthreads();
   cplx sv = mk(0.0), su = mk(0.0), g = mk(0.0), f = mk(0.0);
   const double *tb = a.tab + (size_t)kk * nf * nf;
-  for(int e = t; e < nf * nf; e += blockDim.x) {
-    int p = e / nf, q = e % nf;
+  // every table entry carries the Clebsch-Gordan factor <J M | J1 M1 J2 M2>,
+  // zero unless M1 + M2 = M: only q = (J2, M - M1) is visited for each p
+  int nk, mk_;
+  unflat(kk, nk, mk_);
+  for(int e2 = t; e2 < nf * nmax; e2 += blockDim.x) {
+    const int p = e2 / nmax, n2 = e2 % nmax + 1;
+    int n1, m1;
+    unflat(p, n1, m1);
+    const int m2 = mk_ - m1;
+    if(m2 < -n2 || m2 > n2) continue;
+    const int q = flat(n2, m2), e = p * nf + q;
     double c10 = tb[e], c11 = tb[sz + e], c00 = tb[2 * sz + e], c01 = tb[3 * sz + e];
     double wm = tb[4 * sz + e], w11 = tb[5 * sz + e], w00 = tb[6 * sz + e], w10 = tb[7 * sz + e], w01 = tb[8 * sz + e];
     cplx a1p = A1[p], a0p = A0[p], amq = Am1[q];
@@ -254,11 +263,15 @@ __global__ __launch_bounds__(256) void abssh_kernel(AbsShArgs a) {
   for(int i = 0; i < NQ; ++i) xm1[i] = xp1[i] = mk(0.0);
   const double *W00 = a.tab + 6 * sz, *W11 = a.tab + 5 * sz, *Wm = a.tab + 4 * sz;
   const size_t base = (size_t)kk * nf * nf;
-  for(int e = t; e < nf * nf; e += blockDim.x) {
-    const int p = e / nf, q = e % nf;
-    int n1, m1, n2, m2;
+  // the W tables vanish unless M1 + M2 = M (their Clebsch-Gordan factor):
+  // only q = (J2, M - M1) is visited for each p
+  for(int e2 = t; e2 < nf * nmax; e2 += blockDim.x) {
+    const int p = e2 / nmax, n2 = e2 % nmax + 1;
+    int n1, m1;
     unflat(p, n1, m1);
-    unflat(q, n2, m2);
+    const int m2 = m - m1;
+    if(m2 < -n2 || m2 > n2) continue;
+    const int q = flat(n2, m2), e = p * nf + q;
     const double w00 = W00[base + e], w11 = W11[base + e], wm = Wm[base + e];
     const double s4 = sqrt((double)n1 * n2 * (n1 + 1) * (n2 + 1));
     const cplx cc = cs[p] * cs[q], dd = ds[p] * ds[q];

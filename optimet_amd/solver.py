@@ -46,7 +46,7 @@ class Stats(ctypes.Structure):
                 ("lu_gemm_flop", ctypes.c_double), ("lu_gemm_launches", ctypes.c_int),
                 ("t_lu_panel_ms", ctypes.c_double), ("t_factor_total_ms", ctypes.c_double),
                 ("pivot_fallbacks", ctypes.c_int), ("sweep_batch", ctypes.c_int), ("matvec_calls", ctypes.c_int),
-                ("t_matvec_ms", ctypes.c_double)]
+                ("t_matvec_ms", ctypes.c_double), ("matvec_bytes", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
