@@ -17,6 +17,8 @@
  *       getScatteringCrossSection_SH (Result.cpp:557-761);
  *   omx_absorption_cross_section_sh         <- Result::getAbsorptionCrossSection_SH
  *       (Result.cpp:763-794);
+ *   omx_fields                              <- Result::getEHFields / setFields
+ *       (Result.cpp:74-301, 797-933);
  *   omx_scattering_size                     <- AbstractSolver::scattering_size()
  *       (Solver.h:132-135).
  *
@@ -161,6 +163,14 @@ int omx_cross_sections(omx_context *ctx, double *out4);
  * Gauss-Legendre integral of |E(2 omega)|^2 inside each sphere times the SH
  * conductivity -i eps0 2 omega (eps_r_SH - 1). */
 int omx_absorption_cross_section_sh(omx_context *ctx, double *out);
+
+/* Field maps of the last solve (Result::getEHFields with the projection off
+ * and Result::setFields, Result.cpp:74-301, 797-933): E and H at omega, and
+ * with E_sh and H_sh non-NULL (SH_cond) at 2 omega, at npts points xyz[3*npts]
+ * (Cartesian, metres), each output 3*npts complex (x, y, z components).
+ * Outside the spheres: incident plus scattered waves; inside a sphere: its
+ * internal waves and, at 2 omega, the bulk-source term (Geometry::COEFFpartSH). */
+int omx_fields(omx_context *ctx, int npts, const double *xyz, double *E, double *H, double *E_sh, double *H_sh);
 
 /* 2 nmax (nmax+2) P, AbstractSolver::scattering_size(). */
 int omx_scattering_size(const omx_context *ctx);

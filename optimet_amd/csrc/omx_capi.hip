@@ -102,7 +102,8 @@ struct omx_context {
   int cg_nmax = 0;
   // device buffers
   DBuf d_xyz, d_radius, d_eps, d_mu, d_eps_sh, d_kppp, d_kpar, d_gam, d_sph, d_ov, d_inc, d_incloc, d_Q, d_w,
-      d_coef, d_S64, d_S32, d_V64, d_V32, d_x, d_xsh, d_xint, d_xintsh, d_K, d_K1, d_cross, d_cg, d_abscoef, d_abssh;
+      d_coef, d_S64, d_S32, d_V64, d_V32, d_x, d_xsh, d_xint, d_xintsh, d_K, d_K1, d_cross, d_cg, d_abscoef, d_abssh,
+      d_fpts, d_fout;
   DBuf d_piv, d_perm, d_cand, d_cnt, d_map, d_status, d_flags, d_wlu;
   hipStream_t aux = nullptr;
   hipEvent_t lu_ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -1170,7 +1171,8 @@ void omx_destroy(omx_context *c) {
                   &c->d_K1, &c->d_cross, &c->d_cg, &c->d_piv, &c->d_perm, &c->d_cand, &c->d_cnt, &c->d_map,
                   &c->d_wlu, &c->d_status, &c->d_flags, &c->d_basis, &c->d_basisZ, &c->d_w1, &c->d_w2, &c->d_r,
                   &c->d_part, &c->d_h, &c->d_nrm, &c->d_gpart, &c->d_piv2, &c->d_perm2, &c->d_flags2,
-                  &c->d_dinv, &c->d_dinv2, &c->d_F64, &c->d_F64sh, &c->d_abscoef, &c->d_abssh};
+                  &c->d_dinv, &c->d_dinv2, &c->d_F64, &c->d_F64sh, &c->d_abscoef, &c->d_abssh,
+                  &c->d_fpts, &c->d_fout};
   for(DBuf *b : bufs) b->release(&c->total);
   auto &w = c->sw;
   DBuf *sbufs[] = {&w.S64, &w.S32, &w.V64, &w.V32, &w.F64, &w.F64sh, &w.eps, &w.eps_sh, &w.ov, &w.incloc, &w.Q,
@@ -1516,6 +1518,42 @@ int omx_absorption_cross_section_sh(omx_context *c, double *out) {
     double s = 0.0;
     for(double v : part) s += v;
     *out = s;
+  });
+}
+
+int omx_fields(omx_context *c, int npts, const double *xyz, double *E, double *H, double *E_sh, double *H_sh) {
+  return guard([&] {
+    if(!c || !c->solved) throw OmxError(OMX_ERR_STATE, "omx_solve must precede omx_fields");
+    if(npts < 1 || !xyz || !E || !H) throw OmxError(OMX_ERR_ARG, "omx_fields: points and E, H outputs required");
+    const bool sh = E_sh || H_sh;
+    if(sh && (!E_sh || !H_sh)) throw OmxError(OMX_ERR_ARG, "omx_fields: E_sh and H_sh go together");
+    if(sh && (!c->sh || c->cg_nmax != c->nmax)) throw OmxError(OMX_ERR_STATE, "no second-harmonic solution");
+    HIPCHK(hipSetDevice(c->device));
+    const int chunk = 1 << 16;
+    const int cap = std::min(npts, chunk);
+    c->d_fpts.ensure(sizeof(double) * 3 * cap, &c->total);
+    c->d_fout.ensure(sizeof(cplx) * 12 * cap, &c->total);
+    for(int i0 = 0; i0 < npts; i0 += chunk) {
+      const int n = std::min(chunk, npts - i0);
+      cplx *o = c->d_fout.as<cplx>();
+      HIPCHK(hipMemcpyAsync(c->d_fpts.p, xyz + 3 * (size_t)i0, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->st));
+      omx::FieldArgs fa{c->P, c->nmax, sh ? 1 : 0, n, c->omega(), c->waveK(), kEps0 * c->bg_eps_r, kMu0 * c->bg_mu_r,
+                        c->d_xyz.as<double>(), c->d_radius.as<double>(), c->d_eps.as<cplx>(), c->d_mu.as<cplx>(),
+                        c->d_eps_sh.as<cplx>(), c->d_gam.as<cplx>(), sh ? c->d_cg.as<double>() : nullptr,
+                        c->d_inc.as<cplx>(), c->d_x.as<cplx>(), c->d_xint.as<cplx>(), c->d_xsh.as<cplx>(),
+                        c->d_xintsh.as<cplx>(), c->d_fpts.as<double>(), o, o + 3 * (size_t)cap, o + 6 * (size_t)cap,
+                        o + 9 * (size_t)cap};
+      omx::run_fields(fa, c->st);
+      HIPCHK(hipGetLastError());
+      const size_t b = sizeof(cplx) * 3 * n, off = 6 * (size_t)i0;
+      HIPCHK(hipMemcpyAsync(E + off, o, b, hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipMemcpyAsync(H + off, o + 3 * (size_t)cap, b, hipMemcpyDeviceToHost, c->st));
+      if(sh) {
+        HIPCHK(hipMemcpyAsync(E_sh + off, o + 6 * (size_t)cap, b, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(H_sh + off, o + 9 * (size_t)cap, b, hipMemcpyDeviceToHost, c->st));
+      }
+      HIPCHK(hipStreamSynchronize(c->st));
+    }
   });
 }
 

@@ -1,5 +1,7 @@
 // omx_incident.hip -- plane-wave coefficients a_inc, b_inc of the excitation
-// (Excitation.cpp:48-75 with AuxCoefficients.cpp:31-343).
+// (Excitation.cpp:48-75 with AuxCoefficients.cpp:31-343), and the field maps
+// (Result::getEHFields, Result.cpp:74-301), which evaluate the same vector
+// spherical waves at every output point.
 //
 // This translation unit is compiled without floating-point contraction
 // (-ffp-contract=off, see optimet_amd/build.py).  For propagation along the z
@@ -16,22 +18,11 @@
 
 namespace omx {
 
-// ------------------------------------------------------------------ incident
-// One block; thread t handles m = t - nmax.  Writes inc[0..nf) = a_inc and
-// inc[nf..2nf) = b_inc.
-__global__ void incident_kernel(IncArgs a) {
-  const int nmax = a.nmax, nf = nmax * (nmax + 2);
-  int t = threadIdx.x;
-  if(t > 2 * nmax) return;
-  int m = t - nmax;
-  const double the = a.theta, phi = a.phi;
-  // Einc = toProjection((0, theta, phi), (0, Etheta, Ephi)) (Reader.cpp:820-827)
-  double st = sin(the), ct = cos(the), sp = sin(phi), cp = cos(phi);
-  cplx E0 = ct * cp * a.etheta - sp * a.ephi;
-  cplx E1 = ct * sp * a.etheta + cp * a.ephi;
-  cplx E2 = -st * a.etheta;
-  // VIGdVIG (AuxCoefficients.cpp:216-290)
-  double W[kMaxNmax + 2], dW[kMaxNmax + 2];
+// ------------------------------------------------------------------ Wigner d
+// VIGdVIG (AuxCoefficients.cpp:216-290) for order m at polar angle the:
+// W[n], dW[n] for n = |m| .. nmax; the reference's operation order and its
+// shift of the polar angle by 1e-6 on the z axis.
+__device__ inline void vigdvig(int nmax, int m, double the, double (&W)[kMaxNmax + 2], double (&dW)[kMaxNmax + 2]) {
   for(int i = 0; i <= nmax + 1; ++i) W[i] = dW[i] = 0.0;
   bool neg = m < 0;
   int am = abs(m);
@@ -70,6 +61,27 @@ __global__ void incident_kernel(IncArgs a) {
       W[i] *= c;
       dW[i] *= -c;
     }
+}
+
+// ------------------------------------------------------------------ incident
+// One block; thread t handles m = t - nmax.  Writes inc[0..nf) = a_inc and
+// inc[nf..2nf) = b_inc.
+__global__ void incident_kernel(IncArgs a) {
+  const int nmax = a.nmax, nf = nmax * (nmax + 2);
+  int t = threadIdx.x;
+  if(t > 2 * nmax) return;
+  int m = t - nmax;
+  const double the = a.theta, phi = a.phi;
+  // Einc = toProjection((0, theta, phi), (0, Etheta, Ephi)) (Reader.cpp:820-827)
+  double st = sin(the), ct = cos(the), sp = sin(phi), cp = cos(phi);
+  cplx E0 = ct * cp * a.etheta - sp * a.ephi;
+  cplx E1 = ct * sp * a.etheta + cp * a.ephi;
+  cplx E2 = -st * a.etheta;
+  // VIGdVIG (AuxCoefficients.cpp:216-290)
+  double W[kMaxNmax + 2], dW[kMaxNmax + 2];
+  vigdvig(nmax, m, the, W, dW);
+  const int am = abs(m);
+  const bool axis = (fabs(the) < 1e-10) || (fabs(the) - kPi + 1e-10 > 0.0);
   double sgn = (am % 2 == 0) ? 1.0 : -1.0;
   cplx ph = mk(cos((double)m * phi), -sin((double)m * phi));
   for(int n = (am > 1 ? am : 1); n <= nmax; ++n) {
@@ -96,6 +108,245 @@ __global__ void incident_kernel(IncArgs a) {
     a.inc[p] = ((((4 * kPi) * sgn) * in) * dn) * dotC * ph;
     a.inc[nf + p] = ((((4 * kPi) * sgn) * in1) * dn) * dotB * ph;
   }
+}
+
+// ------------------------------------------------------------------ fields
+__device__ inline void to_cart(double the, double phi, const cplx v[3], cplx out[3]) {
+  // Tools::toProjection
+  const double st = sin(the), ct = cos(the), sp = sin(phi), cp = cos(phi);
+  out[0] = st * cp * v[0] + ct * cp * v[1] - sp * v[2];
+  out[1] = st * sp * v[0] + ct * sp * v[1] + cp * v[2];
+  out[2] = ct * v[0] - st * v[1];
+}
+__device__ inline void to_sph(double x, double y, double z, double &r, double &t, double &p) {
+  // Tools::toSpherical
+  r = sqrt(x * x + y * y + z * z);
+  t = r > 0.0 ? acos(z / r) : 0.0;
+  p = r > 0.0 ? atan2(y, x) : 0.0;
+}
+
+// Sums over the harmonics of one expansion about a centre, at the point
+// (Rr, Rt, Rp) relative to it, of the vector spherical waves of
+// AuxCoefficients (AuxCoefficients.cpp:30-214): sE = sum M_p a_p + N_p b_p,
+// sH = sum N_p a_p + M_p b_p, Cartesian.  Regular (j_n) or radiating (h_n^(1)).
+__device__ void wave_sum(int nmax, double Rr, double Rt, double Rp, cplx k, bool regular, const cplx *a,
+                         const cplx *b, cplx sE[3], cplx sH[3]) {
+  cplx f[kMaxNmax + 2], d[kMaxNmax + 2];
+  sph_bessel(regular ? 0 : 1, Rr * k, nmax, f, d);
+  const cplx Kr = Rr * k, iKr = mk(1.0) / Kr;
+  const bool axis = (fabs(Rt) < 1e-10) || (fabs(Rt) - kPi + 1e-10 > 0.0);
+  cplx e3[3] = {mk(0.0), mk(0.0), mk(0.0)}, h3[3] = {mk(0.0), mk(0.0), mk(0.0)};
+  for(int m = -nmax; m <= nmax; ++m) {
+    double W[kMaxNmax + 2], dW[kMaxNmax + 2];
+    vigdvig(nmax, m, Rt, W, dW);
+    const double dm = (m % 2 == 0) ? 1.0 : -1.0;  // Legendre to Wigner
+    const cplx e = mk(cos(m * Rp), sin(m * Rp));
+    for(int n = (abs(m) > 1 ? abs(m) : 1); n <= nmax; ++n) {
+      double A;
+      if(m == 0)
+        A = 0.0;
+      else if(axis)
+        A = m / cos(Rt) * dW[n];
+      else
+        A = m / sin(Rt) * W[n];
+      const double dn = sqrt((2.0 * n + 1.0) / (4.0 * kPi * (n * (n + 1))));
+      const cplx ct = (dm * dn) * e;
+      const cplx z = f[n], zz = Kr * d[n] + f[n];
+      // M = ct z (0, iA, -dW); N = (1/Kr) dm dn (n(n+1) z W, zz dW, zz iA) e
+      const cplx Mth = ct * z * mk(0.0, A), Mph = ct * z * mk(-dW[n]);
+      const cplx Nr = iKr * (dm * dn) * (((double)(n * (n + 1))) * z * W[n]) * e;
+      const cplx Nth = iKr * (dm * dn) * (zz * dW[n]) * e;
+      const cplx Nph = iKr * (dm * dn) * (zz * mk(0.0, A)) * e;
+      const int p = flat(n, m);
+      const cplx ap = a[p], bp = b[p];
+      e3[0] = e3[0] + Nr * bp;
+      e3[1] = e3[1] + Mth * ap + Nth * bp;
+      e3[2] = e3[2] + Mph * ap + Nph * bp;
+      h3[0] = h3[0] + Nr * ap;
+      h3[1] = h3[1] + Nth * ap + Mth * bp;
+      h3[2] = h3[2] + Nph * ap + Mph * bp;
+    }
+  }
+  to_cart(Rt, Rp, e3, sE);
+  to_cart(Rt, Rp, h3, sH);
+}
+
+// symbol::CXm1 / CXp1 (Symbol.cpp:474-655) of object j at radius r for output
+// harmonic kk: the bulk (gamma) source of the fundamental internal field x_int
+// (unconjugated); only the table entries with M1 + M2 = M (all others vanish)
+__device__ void coeff_part_sh(const FieldArgs &a, int j, int kk, double r, cplx &xm1, cplx &xp1) {
+  const int nmax = a.nmax, nf = nmax * (nmax + 2);
+  const size_t sz = (size_t)nf * nf * nf, base = (size_t)kk * nf * nf;
+  const double *W00 = a.tab + 6 * sz, *W11 = a.tab + 5 * sz, *Wm = a.tab + 4 * sz;
+  const cplx kj = a.omega * csqrt_(a.eps[j] * a.mu[j]);
+  const cplx pre = (-mk(kEps0) / a.eps_sh[j]) * a.gamma[j];
+  const cplx ik2 = mk(1.0) / (kj * kj);
+  cplx f[kMaxNmax + 2], d[kMaxNmax + 2], d2[kMaxNmax + 2];
+  const cplx zz = r * kj;
+  sph_bessel(0, zz, nmax, f, d);
+  for(int n = 0; n <= nmax; ++n) {
+    cplx v = mk(0.0);
+    if(n >= 1 && hypot(zz.x, zz.y) > kErrEps) {  // bessel3der (Bessel.h:230-231)
+      v = -(mk((double)n) / zz) * d[n] + (mk((double)n) / (zz * zz)) * f[n] + d[n - 1];
+      if(zz.y == 0.0) v.y = 0.0;
+    }
+    d2[n] = v;
+  }
+  int n, m;
+  unflat(kk, n, m);
+  const double snn = sqrt((double)n * (n + 1)), ir = 1.0 / r;
+  const cplx *c = a.xint + (size_t)j * 2 * nf, *dd = c + nf;
+  xm1 = xp1 = mk(0.0);
+  for(int p = 0; p < nf; ++p) {
+    int n1, m1;
+    unflat(p, n1, m1);
+    for(int n2 = 1; n2 <= nmax; ++n2) {
+      const int m2 = m - m1;
+      if(m2 < -n2 || m2 > n2) continue;
+      const int q = flat(n2, m2);
+      const size_t e = base + (size_t)p * nf + q;
+      const double w00 = W00[e], w11 = W11[e], wm = Wm[e];
+      const double s4 = sqrt((double)n1 * n2 * (n1 + 1) * (n2 + 1));
+      const cplx cc = c[p] * c[q], dq = dd[p] * dd[q];
+      const cplx j1 = f[n1], j2 = f[n2], dj1 = d[n1], dj2 = d[n2];
+      const cplx fd00 = j1 * dj2 * kj + kj * dj1 * j2;
+      const cplx b1 = kj * dj1 + j1 * ir, b2 = kj * dj2 + j2 * ir;
+      const cplx a1 = kj * kj * d2[n1] - (ir * ir) * j1 + ir * kj * dj1;
+      const cplx a2 = kj * kj * d2[n2] - (ir * ir) * j2 + ir * kj * dj2;
+      const cplx fd11 = a1 * b2 + a2 * b1;
+      const cplx fdm = (ir * ir) * (kj * dj1 * j2 + j1 * kj * dj2) - (2.0 * ir * ir * ir) * (j1 * j2);
+      const cplx f00 = j1 * dj2, f11 = b2 * b1, fm = (ir * ir) * (j1 * j2);
+      xm1 = xm1 + pre * (cc * w00 * fd00 + dq * ik2 * (w11 * fd11 + (wm * s4) * fdm));
+      xp1 = xp1 + pre * (cc * (w00 * snn * ir) * f00 + dq * ik2 * ((w11 * snn * ir) * f11 + (wm * s4 * snn * ir) * fm));
+    }
+  }
+}
+
+// One workgroup per output point.  Outside every sphere the threads take the
+// incident wave and the radiating waves of the objects; inside the first
+// sphere that holds the point (Geometry::checkInner, Geometry.cpp:147-163)
+// thread 0 takes its internal waves and the threads share the harmonics of
+// the SH bulk term X(-1) CXm1 + X(+1) CXp1.  The partial fields are summed in
+// LDS in a fixed order.
+__global__ __launch_bounds__(256) void fields_kernel(FieldArgs a) {
+  const int ip = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
+  const int P = a.P, nmax = a.nmax, nf = nmax * (nmax + 2);
+  const double px = a.pts[3 * ip], py = a.pts[3 * ip + 1], pz = a.pts[3 * ip + 2];
+  __shared__ int inside;
+  __shared__ cplx red[12][256];
+  if(t == 0) inside = P;
+  __syncthreads();
+  for(int j = t; j < P; j += nt) {
+    const double dx = px - a.xyz[3 * j], dy = py - a.xyz[3 * j + 1], dz = pz - a.xyz[3 * j + 2];
+    if(sqrt(dx * dx + dy * dy + dz * dz) <= a.radius[j]) atomicMin(&inside, j);
+  }
+  __syncthreads();
+  const int in = inside;
+  cplx acc[12];
+  for(int i = 0; i < 12; ++i) acc[i] = mk(0.0);
+  const cplx k0 = mk(a.omega * sqrt(kEps0 * kMu0)), mi = mk(0.0, -1.0);
+  cplx sE[3], sH[3];
+  if(in == P) {
+    const cplx iZ = mi / csqrt_(a.bg_mu / a.bg_eps);
+    for(int task = t; task <= P; task += nt) {
+      double Rr, Rt, Rp;
+      if(task == P) {  // incident wave about the origin
+        to_sph(px, py, pz, Rr, Rt, Rp);
+        wave_sum(nmax, Rr, Rt, Rp, a.waveK, true, a.inc, a.inc + nf, sE, sH);
+        for(int c = 0; c < 3; ++c) {
+          acc[c] = acc[c] + sE[c];
+          acc[3 + c] = acc[3 + c] + sH[c] * iZ;
+        }
+        continue;
+      }
+      const int j = task;
+      to_sph(px - a.xyz[3 * j], py - a.xyz[3 * j + 1], pz - a.xyz[3 * j + 2], Rr, Rt, Rp);
+      const cplx *x = a.x + (size_t)j * 2 * nf;
+      wave_sum(nmax, Rr, Rt, Rp, a.waveK, false, x, x + nf, sE, sH);
+      for(int c = 0; c < 3; ++c) {
+        acc[c] = acc[c] + sE[c];
+        acc[3 + c] = acc[3 + c] + sH[c] * iZ;
+      }
+      if(a.sh) {
+        const cplx *y = a.xsh + (size_t)j * 2 * nf;
+        wave_sum(nmax, Rr, Rt, Rp, 2.0 * a.waveK, false, y, y + nf, sE, sH);
+        for(int c = 0; c < 3; ++c) {
+          acc[6 + c] = acc[6 + c] + sE[c] * k0;
+          acc[9 + c] = acc[9 + c] + sH[c] * (iZ * k0);
+        }
+      }
+    }
+  } else {
+    const int j = in;
+    double Rr, Rt, Rp;
+    to_sph(px - a.xyz[3 * j], py - a.xyz[3 * j + 1], pz - a.xyz[3 * j + 2], Rr, Rt, Rp);
+    if(t == 0) {
+      const cplx ki = k0 * csqrt_((a.eps[j] / mk(kEps0)) * (a.mu[j] / mk(kMu0)));
+      const cplx iZo = mi / csqrt_(a.mu[j] / a.eps[j]);
+      const cplx *x = a.xint + (size_t)j * 2 * nf;
+      wave_sum(nmax, Rr, Rt, Rp, ki, true, x, x + nf, sE, sH);
+      for(int c = 0; c < 3; ++c) {
+        acc[c] = acc[c] + sE[c];
+        acc[3 + c] = acc[3 + c] + sH[c] * iZo;
+      }
+      if(a.sh) {
+        const cplx ksh = (2.0 * k0) * csqrt_((a.eps_sh[j] / mk(kEps0)) * (a.mu[j] / mk(kMu0)));
+        const cplx iZsh = mi / csqrt_(a.mu[j] / a.eps_sh[j]);
+        const cplx *y = a.xintsh + (size_t)j * 2 * nf;
+        wave_sum(nmax, Rr, Rt, Rp, ksh, true, y, y + nf, sE, sH);
+        for(int c = 0; c < 3; ++c) {
+          acc[6 + c] = acc[6 + c] + sE[c] * k0;
+          acc[9 + c] = acc[9 + c] + sH[c] * (iZsh * k0);
+        }
+      }
+    }
+    if(a.sh) {  // E_gamma: the bulk term's vector harmonics X(-1), X(+1)
+      const bool axis = (fabs(Rt) < 1e-10) || (fabs(Rt) - kPi + 1e-10 > 0.0);
+      for(int kk = t; kk < nf; kk += nt) {
+        cplx xm, xp;
+        coeff_part_sh(a, j, kk, Rr, xm, xp);
+        int n, m;
+        unflat(kk, n, m);
+        double W[kMaxNmax + 2], dW[kMaxNmax + 2];
+        vigdvig(nmax, m, Rt, W, dW);
+        double A;
+        if(m == 0)
+          A = 0.0;
+        else if(axis)
+          A = m / cos(Rt) * dW[n];
+        else
+          A = m / sin(Rt) * W[n];
+        const double dm = (m % 2 == 0) ? 1.0 : -1.0;
+        const double dn = sqrt((2.0 * n + 1.0) / (4.0 * kPi * (n * (n + 1))));
+        const cplx e = mk(cos(m * Rp), sin(m * Rp));
+        // X(-1) = dm dn sqrt(n(n+1)) e (W, 0, 0); X(+1) = dm dn e (0, dW, iA)
+        const cplx v[3] = {(dm * dn * sqrt((double)(n * (n + 1)))) * e * W[n] * xm,
+                           (dm * dn) * e * mk(dW[n]) * xp, (dm * dn) * e * mk(0.0, A) * xp};
+        cplx g[3];
+        to_cart(Rt, Rp, v, g);
+        for(int c = 0; c < 3; ++c) acc[6 + c] = acc[6 + c] + g[c];
+      }
+    }
+  }
+  for(int i = 0; i < 12; ++i) red[i][t] = acc[i];
+  __syncthreads();
+  for(int st = nt / 2; st > 0; st >>= 1) {
+    if(t < st)
+      for(int i = 0; i < 12; ++i) red[i][t] = red[i][t] + red[i][t + st];
+    __syncthreads();
+  }
+  if(t < 3) {
+    a.E[3 * ip + t] = red[t][0];
+    a.H[3 * ip + t] = red[3 + t][0];
+    if(a.sh) {
+      a.Esh[3 * ip + t] = red[6 + t][0];
+      a.Hsh[3 * ip + t] = red[9 + t][0];
+    }
+  }
+}
+
+void run_fields(const FieldArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL(fields_kernel, dim3(a.npts), dim3(256), 0, st, a);
 }
 
 void run_incident(const IncArgs &a, hipStream_t st) {

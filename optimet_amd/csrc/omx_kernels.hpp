@@ -82,6 +82,22 @@ struct CrossArgs {
   double *out;                       // [P][4]
 };
 
+// Result::getEHFields / setFields (Result.cpp:74-301, 797-933): E and H at
+// omega and 2 omega at npts Cartesian points, Cartesian components.
+struct FieldArgs {
+  int P, nmax, sh, npts;
+  double omega;
+  cplx waveK, bg_eps, bg_mu;               // background wavenumber; absolute eps, mu
+  const double *xyz, *radius;              // object centres (Cartesian) and radii
+  const cplx *eps, *mu, *eps_sh, *gamma;   // absolute
+  const double *tab;                       // nine tables (SH bulk term inside spheres)
+  const cplx *inc;                         // [a_inc; b_inc]
+  const cplx *x, *xint, *xsh, *xintsh;     // solution vectors (N each)
+  const double *pts;                       // 3 npts
+  cplx *E, *H, *Esh, *Hsh;                 // 3 npts each; Esh, Hsh unused without sh
+};
+void run_fields(const FieldArgs &a, hipStream_t st);
+
 // nb > 1: a batch of nb systems in one launch (the batch fields of the args)
 void run_objects(const ObjArgs &a, hipStream_t st, int nb = 1);
 void run_incident(const IncArgs &a, hipStream_t st);

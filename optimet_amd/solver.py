@@ -28,7 +28,8 @@ EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "o
            "omx_set_solver", "omx_set_cg_tables", "omx_build_cg_tables", "omx_update", "omx_solve",
            "omx_cross_sections", "omx_scattering_size", "omx_get_stats", "omx_read_matrix", "omx_read_vector",
            "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_set_lu32_gemm", "omx_solve_sweep", "omx_mem_info", "omx_trim",
-           "omx_dense_solve_noalloc", "omx_read_matrix_block", "omx_absorption_cross_section_sh"]
+           "omx_dense_solve_noalloc", "omx_read_matrix_block", "omx_absorption_cross_section_sh",
+           "omx_fields"]
 
 
 class OmxError(RuntimeError):
@@ -91,6 +92,7 @@ def load_library(path=LIB_PATH):
     L.omx_dense_solve_noalloc.argtypes = [P, I, V, ctypes.c_size_t, V, V]
     L.omx_read_matrix_block.argtypes = [P, I, I, I, I, I, D]
     L.omx_absorption_cross_section_sh.argtypes = [P, D]
+    L.omx_fields.argtypes = [P, I, D, D, D, D, D]
     _lib = L
     return L
 
@@ -269,6 +271,18 @@ class Solver:
         if self.cluster.sh:
             _check(self.ctx.L.omx_absorption_cross_section_sh(self.ctx.h, _d(cs[4:])))
         return {"cext": cs[0], "csca": cs[1], "cabs": cs[2], "csca_sh": cs[3], "cabs_sh": cs[4]}
+
+    def fields(self, xyz, sh=None):
+        """E and H (Cartesian components) at points xyz (n, 3) in metres, at
+        omega and, for a second-harmonic solve (or sh=True), at 2 omega:
+        dict E, H (and E_sh, H_sh), each (n, 3) complex (omx_fields)."""
+        pts = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
+        n = pts.shape[0]
+        sh = self.cluster.sh if sh is None else sh
+        out = {k: np.zeros((n, 3), np.complex128) for k in (("E", "H", "E_sh", "H_sh") if sh else ("E", "H"))}
+        ptr = lambda k: _d(out[k].view(np.float64)) if k in out else None
+        _check(self.ctx.L.omx_fields(self.ctx.h, n, _d(pts), ptr("E"), ptr("H"), ptr("E_sh"), ptr("H_sh")))
+        return out
 
     def matrix(self, sh=False):
         N = self.cluster.N

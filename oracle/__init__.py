@@ -84,6 +84,7 @@ def _declare(L):
     L.ora_solve_all.argtypes = [_P, _I, _D, _D, _D, _D, _D]
     L.ora_cross_sections.argtypes = [_P, _D, _D, _D]
     L.ora_absorption_sh.argtypes = [_P, _D, _D, _D, _D]
+    L.ora_fields.argtypes = [_P, _D, _D, _D, _D, _D, _I, _D, _D, _D, _D, _D]
     L.ora_time_sample.argtypes = [_P, _I, _I, _I, _D]
 
 
@@ -320,6 +321,22 @@ def absorption_sh(prob, tables, x_int, x_int_sh):
     p = _Pinned(prob)
     _check(lib().ora_absorption_sh(p.ref(), _ptr(t), _ptr(a.view(np.float64)), _ptr(b.view(np.float64)), _ptr(out)))
     return float(out[0])
+
+
+def fields(prob, tables, sol, xyz):
+    """Result::getEHFields (Cartesian components) at points xyz (n, 3):
+    dict E, H, E_sh, H_sh, each (n, 3) complex; sol holds x_sca, x_int and,
+    with the second harmonic, x_sca_sh, x_int_sh."""
+    pts = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
+    n = pts.shape[0]
+    outs = [_cz(3 * n) for _ in range(4)]
+    t = np.ascontiguousarray(tables, dtype=np.float64).reshape(-1)
+    keep = {k: np.ascontiguousarray(sol[k], dtype=np.complex128) for k in sol if k.startswith("x_")}
+    ptr = lambda k: _ptr(keep[k].view(np.float64)) if k in keep else None
+    p = _Pinned(prob)
+    _check(lib().ora_fields(p.ref(), _ptr(t), ptr("x_sca"), ptr("x_int"), ptr("x_sca_sh"), ptr("x_int_sh"), n,
+                            _ptr(pts), *[_ptr(o.view(np.float64)) for o in outs]))
+    return {k: o.reshape(n, 3) for k, o in zip(("E", "H", "E_sh", "H_sh"), outs)}
 
 
 def time_sample(prob, npairs, nsolve, method=1):

@@ -1001,6 +1001,51 @@ void bessel_d2(cd z, int nmax, std::vector<cd> &d, std::vector<cd> &dd, std::vec
   }
 }
 
+// symbol::CXm1 / CXp1 (Symbol.cpp:474-655), the sums COEFFXm1 / COEFFXp1 of
+// ACSshcoeff (:430-455): coefficients of the X(-1) and X(+1) vector harmonics
+// of the bulk (gamma) source of the fundamental internal field of object obj
+// at radius r, for output harmonic kk.
+void coeff_part_sh(const Problem &pb, const double *tab, const cd *x_int, int obj, int kk, double r, cd &xm1,
+                   cd &xp1) {
+  const int nmax = pb.nmax, nf = pb.nf;
+  const size_t sz = (size_t)nf * nf * nf;
+  const double *Wm1m1 = tab + 4 * sz, *W11 = tab + 5 * sz, *W00 = tab + 6 * sz;
+  const Object &o = pb.obj[obj];
+  const cd eps_0 = kEps0;
+  const cd kj = pb.omega * std::sqrt(o.eps * o.mu);
+  const cd pre = (-eps_0 / o.eps_sh) * o.gamma;
+  const cd *c = x_int + (size_t)obj * 2 * nf, *d = c + nf;
+  const int n = (int)std::sqrt(kk + 1.0);
+  const double snn = std::sqrt((double)n * (n + 1));
+  std::vector<cd> j, dj, d2j;
+  bessel_d2(r * kj, nmax, j, dj, d2j);
+  xm1 = 0.0;
+  xp1 = 0.0;
+  for(int p = 0; p < nf; ++p) {
+    const int n1 = (int)std::sqrt(p + 1.0);
+    for(int q = 0; q < nf; ++q) {
+      const int n2 = (int)std::sqrt(q + 1.0);
+      const size_t t = (size_t)kk * nf * nf + (size_t)p * nf + q;
+      const double w00 = W00[t], w11 = W11[t], wm = Wm1m1[t];
+      const double s4 = std::sqrt((double)n1 * n2 * (n1 + 1) * (n2 + 1));
+      const cd cc = c[p] * c[q], ddp = d[p] * d[q];
+      // F_d00, F_d11, F_dm1m1, F_00, F_11, F_m1m1 (Symbol.cpp:80-141)
+      const cd fd00 = j[n1] * dj[n2] * kj + kj * dj[n1] * j[n2];
+      const cd a1 = kj * kj * d2j[n1] - (1.0 / (r * r)) * j[n1] + (1.0 / r) * kj * dj[n1];
+      const cd a2 = kj * kj * d2j[n2] - (1.0 / (r * r)) * j[n2] + (1.0 / r) * kj * dj[n2];
+      const cd b1 = kj * dj[n1] + j[n1] / r, b2 = kj * dj[n2] + j[n2] / r;
+      const cd fd11 = a1 * b2 + a2 * b1;
+      const cd fdm1m1 = (1.0 / (r * r)) * (kj * dj[n1] * j[n2] + j[n1] * kj * dj[n2]) - (2.0 / (r * r * r)) * (j[n1] * j[n2]);
+      const cd f00 = j[n1] * dj[n2];
+      const cd f11 = b2 * b1;
+      const cd fm1m1 = (1.0 / (r * r)) * (j[n1] * j[n2]);
+      xm1 += pre * (cc * w00 * fd00 + ddp * (1.0 / (kj * kj)) * (w11 * fd11 + wm * s4 * fdm1m1));
+      xp1 += pre * (cc * w00 * snn * (1.0 / r) * f00 +
+                    ddp * (1.0 / (kj * kj)) * (w11 * snn * (1.0 / r) * f11 + wm * s4 * snn * (1.0 / r) * fm1m1));
+    }
+  }
+}
+
 // Geometry::AbsCSSHcoeff -> symbol::ACSshcoeff (Geometry.cpp:428-455,
 // Symbol.cpp:358-470) for every output harmonic kk of object obj: the
 // 4-point Gauss-Legendre radial integral over [0, R] of |E(2 omega)|^2 built
@@ -1010,55 +1055,26 @@ void bessel_d2(cd z, int nmax, std::vector<cd> &d, std::vector<cd> &dd, std::vec
 // (Symbol.cpp:460), kept literally.
 cd abs_sh_object(const Problem &pb, const double *tab, const cd *x_int, const cd *x_int_sh, int obj) {
   const int nmax = pb.nmax, nf = pb.nf;
-  const size_t sz = (size_t)nf * nf * nf;
-  const double *Wm1m1 = tab + 4 * sz, *W11 = tab + 5 * sz, *W00 = tab + 6 * sz;
   const Object &o = pb.obj[obj];
   const double R = o.radius, omega = pb.omega;
   const cd eps_0 = kEps0, mu_0 = kMu0;
-  const cd k01 = omega * std::sqrt(eps_0 * mu_0), kj = omega * std::sqrt(o.eps * o.mu);
+  const cd k01 = omega * std::sqrt(eps_0 * mu_0);
   const cd ksh = 2.0 * omega * std::sqrt(o.eps_sh * o.mu_sh);
-  const cd pre = (-eps_0 / o.eps_sh) * o.gamma;
   // Gauss-Legendre nodes and weights on [-1, 1] (Symbol.cpp:397-398)
   const double xi[4] = {-0.3399810435848563, 0.3399810435848563, -0.8611363115940526, 0.8611363115940526};
   const double wi[4] = {0.6521451548625461, 0.6521451548625461, 0.3478548451374538, 0.3478548451374538};
-  const cd *c = x_int + (size_t)obj * 2 * nf, *d = c + nf;
   const cd *csh = x_int_sh + (size_t)obj * 2 * nf, *dsh = csh + nf;
-  std::vector<int> nof(nf);
-  for(int p = 0; p < nf; ++p) nof[p] = (int)std::sqrt(p + 1.0);
   cd total = 0.0;
   for(int kk = 0; kk < nf; ++kk) {
-    const int n = nof[kk];
+    const int n = (int)std::sqrt(kk + 1.0);
     const double snn = std::sqrt((double)n * (n + 1));
     cd integral = 0.0;
     for(int ii = 0; ii < 4; ++ii) {
       const double r = (R / 2.0) * xi[ii] + R / 2.0;
-      std::vector<cd> j, dj, d2j, jsh, djsh;
-      bessel_d2(r * kj, nmax, j, dj, d2j);
+      std::vector<cd> jsh, djsh;
       bessel(kBessel, r * ksh, nmax, jsh, djsh);
-      cd xm1 = 0.0, xp1 = 0.0;
-      for(int p = 0; p < nf; ++p) {
-        const int n1 = nof[p];
-        for(int q = 0; q < nf; ++q) {
-          const int n2 = nof[q];
-          const size_t t = (size_t)kk * nf * nf + (size_t)p * nf + q;
-          const double w00 = W00[t], w11 = W11[t], wm = Wm1m1[t];
-          const double s4 = std::sqrt((double)n1 * n2 * (n1 + 1) * (n2 + 1));
-          const cd cc = c[p] * c[q], ddp = d[p] * d[q];
-          // F_d00, F_d11, F_dm1m1, F_00, F_11, F_m1m1 (Symbol.cpp:80-141)
-          const cd fd00 = j[n1] * dj[n2] * kj + kj * dj[n1] * j[n2];
-          const cd a1 = kj * kj * d2j[n1] - (1.0 / (r * r)) * j[n1] + (1.0 / r) * kj * dj[n1];
-          const cd a2 = kj * kj * d2j[n2] - (1.0 / (r * r)) * j[n2] + (1.0 / r) * kj * dj[n2];
-          const cd b1 = kj * dj[n1] + j[n1] / r, b2 = kj * dj[n2] + j[n2] / r;
-          const cd fd11 = a1 * b2 + a2 * b1;
-          const cd fdm1m1 = (1.0 / (r * r)) * (kj * dj[n1] * j[n2] + j[n1] * kj * dj[n2]) - (2.0 / (r * r * r)) * (j[n1] * j[n2]);
-          const cd f00 = j[n1] * dj[n2];
-          const cd f11 = b2 * b1;
-          const cd fm1m1 = (1.0 / (r * r)) * (j[n1] * j[n2]);
-          xm1 += pre * (cc * w00 * fd00 + ddp * (1.0 / (kj * kj)) * (w11 * fd11 + wm * s4 * fdm1m1));
-          xp1 += pre * (cc * w00 * snn * (1.0 / r) * f00 +
-                        ddp * (1.0 / (kj * kj)) * (w11 * snn * (1.0 / r) * f11 + wm * s4 * snn * (1.0 / r) * fm1m1));
-        }
-      }
+      cd xm1, xp1;
+      coeff_part_sh(pb, tab, x_int, obj, kk, r, xm1, xp1);
       const cd xm1sh = k01 * dsh[kk] * (1.0 / ksh) * snn * (1.0 / r) * jsh[n];
       const cd x0 = -k01 * csh[kk] * jsh[n];
       const cd xp1sh = k01 * dsh[kk] * (1.0 / ksh) * ((1.0 / r) * jsh[n] + djsh[n]);
@@ -1083,6 +1099,135 @@ double abs_sh(const Problem &pb, const double *tab, const cd *x_int, const cd *x
   double s = 0.0;
   for(double v : part) s += v;
   return s;
+}
+
+// ---------------------------------------------------------------- fields
+// AuxCoefficients (AuxCoefficients.cpp:30-343): the vector spherical waves M,
+// N and the vector harmonics X(-1), X(+1) of every harmonic p at the point R
+// (spherical, relative to the expansion centre), projected on Cartesian axes
+// (Tools::toProjection); regular (j_n) or radiating (h_n^(1)).
+struct Vswf {
+  std::vector<cd> M, N, Xm, Xp;  // [3 nf]: (x, y, z) of harmonic p at 3p
+};
+void vswf(Sph R, cd k, bool regular, int nmax, Vswf &v) {
+  const int nf = nmax * (nmax + 2);
+  v.M.assign(3 * nf, 0.0);
+  v.N.assign(3 * nf, 0.0);
+  v.Xm.assign(3 * nf, 0.0);
+  v.Xp.assign(3 * nf, 0.0);
+  std::vector<double> dn(nmax + 1);
+  dn[0] = -1000;
+  for(int i = 1; i <= nmax; ++i) dn[i] = std::sqrt((2.0 * i + 1.0) / (4.0 * kPi * (i * (i + 1))));
+  std::vector<cd> data, ddata;
+  bessel(regular ? kBessel : kHankel1, R.r * k, nmax, data, ddata);
+  const bool axis = std::abs(R.the) < 1e-10 || (std::abs(R.the) - kPi + 1e-10) > 0.0;
+  const cd Kr = k * R.r;
+  for(int m = -nmax; m <= nmax; ++m) {
+    std::vector<double> W, dW;
+    vigdvig(nmax, m, R, W, dW);
+    const double dm = (m % 2 == 0) ? 1.0 : -1.0;  // Legendre to Wigner
+    const cd e(std::cos(m * R.phi), std::sin(m * R.phi));
+    for(int n = std::max(1, std::abs(m)); n <= nmax; ++n) {
+      double A;
+      if(m == 0)
+        A = 0.0;
+      else if(axis)
+        A = m / std::cos(R.the) * dW[n];
+      else
+        A = m / std::sin(R.the) * W[n];
+      const cd Pn[3] = {W[n], 0.0, 0.0}, Cn[3] = {0.0, cd(0.0, A), cd(-dW[n], 0.0)},
+               Bn[3] = {0.0, cd(dW[n], 0.0), cd(0.0, A)};
+      const cd ct = dm * dn[n] * e;
+      cd Mv[3], Nv[3], Xmv[3], Xpv[3];
+      for(int c = 0; c < 3; ++c) {
+        Mv[c] = ct * data[n] * Cn[c];
+        Nv[c] = (1.0 / Kr) * dm * dn[n] * (((double)(n * (n + 1))) * data[n] * Pn[c] + (Kr * ddata[n] + data[n]) * Bn[c]) * e;
+        Xmv[c] = dm * dn[n] * std::sqrt((double)(n * (n + 1))) * e * Pn[c];
+        Xpv[c] = dm * dn[n] * e * Bn[c];
+      }
+      const int p = flat(n, m);
+      to_projection(R, Mv, &v.M[3 * p]);
+      to_projection(R, Nv, &v.N[3 * p]);
+      to_projection(R, Xmv, &v.Xm[3 * p]);
+      to_projection(R, Xpv, &v.Xp[3 * p]);
+    }
+  }
+}
+
+// Result::getEHFields with projection off (Result.cpp:74-301), and the
+// particular solution coefficients of Result::setFields (Geometry.cpp:457-490)
+// inside a sphere: E and H at omega and 2 omega at the point xyz (Cartesian),
+// as Cartesian components.  Outside every sphere: the incident wave plus the
+// radiating waves of every object (at 2 omega, the SH scattered waves, scaled
+// by the vacuum wavenumber); inside the first sphere that holds the point
+// (Geometry::checkInner): its internal waves (and, at 2 omega, the bulk source
+// term X(-1) CXm1 + X(+1) CXp1).
+void fields_at(const Problem &pb, const double *tab, const cd *xs, const cd *xi, const cd *xss, const cd *xis,
+               const double xyz[3], cd E[3], cd H[3], cd Esh[3], cd Hsh[3]) {
+  const int nmax = pb.nmax, nf = pb.nf;
+  for(int c = 0; c < 3; ++c) E[c] = H[c] = Esh[c] = Hsh[c] = 0.0;
+  const Sph R = to_spherical({xyz[0], xyz[1], xyz[2]});
+  const cd mi(0.0, -1.0);
+  const cd k0 = pb.omega * std::sqrt(cd(kEps0) * cd(kMu0));
+  const cd iZ = mi / std::sqrt(pb.bg_mu / pb.bg_eps);
+  int in = -1;
+  for(int j = 0; j < pb.P && in < 0; ++j)
+    if(sph_minus(R, pb.obj[j].vR).r <= pb.obj[j].radius) in = j;
+  Vswf v;
+  if(in < 0) {
+    vswf(R, pb.waveK, true, nmax, v);
+    for(int p = 0; p < nf; ++p)
+      for(int c = 0; c < 3; ++c) {
+        E[c] += v.M[3 * p + c] * pb.ap[p] + v.N[3 * p + c] * pb.bp[p];
+        H[c] += (v.N[3 * p + c] * pb.ap[p] + v.M[3 * p + c] * pb.bp[p]) * iZ;
+      }
+    for(int j = 0; j < pb.P; ++j) {
+      const Sph Rr = sph_minus(R, pb.obj[j].vR);
+      vswf(Rr, pb.waveK, false, nmax, v);
+      const cd *x = xs + (size_t)j * 2 * nf;
+      for(int p = 0; p < nf; ++p)
+        for(int c = 0; c < 3; ++c) {
+          E[c] += v.M[3 * p + c] * x[p] + v.N[3 * p + c] * x[nf + p];
+          H[c] += (v.N[3 * p + c] * x[p] + v.M[3 * p + c] * x[nf + p]) * iZ;
+        }
+      if(pb.sh && xss) {
+        vswf(Rr, 2.0 * pb.waveK, false, nmax, v);
+        const cd *y = xss + (size_t)j * 2 * nf;
+        for(int p = 0; p < nf; ++p)
+          for(int c = 0; c < 3; ++c) {
+            Esh[c] += (v.M[3 * p + c] * y[p] + v.N[3 * p + c] * y[nf + p]) * k0;
+            Hsh[c] += (v.N[3 * p + c] * y[p] + v.M[3 * p + c] * y[nf + p]) * (iZ * k0);
+          }
+      }
+    }
+    return;
+  }
+  const Object &o = pb.obj[in];
+  const Sph Rr = sph_minus(R, o.vR);
+  const cd ki = k0 * std::sqrt(o.eps / kEps0 * (o.mu / kMu0));
+  const cd iZo = mi / std::sqrt(o.mu / o.eps);
+  vswf(Rr, ki, true, nmax, v);
+  const cd *x = xi + (size_t)in * 2 * nf;
+  for(int p = 0; p < nf; ++p)
+    for(int c = 0; c < 3; ++c) {
+      E[c] += v.M[3 * p + c] * x[p] + v.N[3 * p + c] * x[nf + p];
+      H[c] += (v.N[3 * p + c] * x[p] + v.M[3 * p + c] * x[nf + p]) * iZo;
+    }
+  if(pb.sh && xis) {
+    const cd ksh = 2.0 * k0 * std::sqrt(o.eps_r_sh * (o.mu_sh / kMu0));
+    const cd iZsh = mi / std::sqrt(o.mu_sh / o.eps_sh);
+    vswf(Rr, ksh, true, nmax, v);
+    const cd *y = xis + (size_t)in * 2 * nf;
+    for(int p = 0; p < nf; ++p) {
+      cd xm, xp;
+      coeff_part_sh(pb, tab, xi, in, p, Rr.r, xm, xp);
+      for(int c = 0; c < 3; ++c) {
+        Esh[c] += (v.M[3 * p + c] * y[p] + v.N[3 * p + c] * y[nf + p]) * k0;
+        Esh[c] += v.Xm[3 * p + c] * xm + v.Xp[3 * p + c] * xp;
+        Hsh[c] += (v.N[3 * p + c] * y[p] + v.M[3 * p + c] * y[nf + p]) * iZsh * k0;
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------- cross sections
@@ -1427,6 +1572,25 @@ int ora_absorption_sh(const ora_problem *in, const double *tables, const double 
   return guard([&] {
     Problem pb = make_problem(in);
     *out = abs_sh(pb, tables, (const cd *)x_int, (const cd *)x_int_sh);
+  });
+}
+
+int ora_fields(const ora_problem *in, const double *tables, const double *x_sca, const double *x_int,
+               const double *x_sca_sh, const double *x_int_sh, int npts, const double *xyz, double *E, double *H,
+               double *Esh, double *Hsh) {
+  return guard([&] {
+    Problem pb = make_problem(in);
+    parallel_for(npts, [&](int i) {
+      cd e[3], h[3], es[3], hs[3];
+      fields_at(pb, tables, (const cd *)x_sca, (const cd *)x_int, (const cd *)x_sca_sh, (const cd *)x_int_sh,
+                xyz + 3 * i, e, h, es, hs);
+      for(int c = 0; c < 3; ++c) {
+        ((cd *)E)[3 * i + c] = e[c];
+        ((cd *)H)[3 * i + c] = h[c];
+        ((cd *)Esh)[3 * i + c] = es[c];
+        ((cd *)Hsh)[3 * i + c] = hs[c];
+      }
+    });
   });
 }
 

@@ -119,6 +119,13 @@ int ora_cross_sections(const ora_problem *pb, const double *x_sca, const double 
 int ora_absorption_sh(const ora_problem *pb, const double *tables, const double *x_int,
                       const double *x_int_sh, double *out);
 
+/* Result::getEHFields at npts Cartesian points xyz[3*npts] (projection off):
+ * E, H at omega and Esh, Hsh at 2 omega, 3*npts complex each (x, y, z).
+ * tables: the nine tables of ora_cg_tables (the SH bulk term inside spheres). */
+int ora_fields(const ora_problem *pb, const double *tables, const double *x_sca, const double *x_int,
+               const double *x_sca_sh, const double *x_int_sh, int npts, const double *xyz, double *E, double *H,
+               double *Esh, double *Hsh);
+
 /* CPU-baseline timing sample: out[0] s per pair coupling + block, out[1] s
  * of one dense solve of order nsolve (method as ora_dense_solve), out[2] s of
  * one object's second-harmonic source. */

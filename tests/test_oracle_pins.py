@@ -584,3 +584,44 @@ def test_sh_absorption_signs_and_scaling(ora):
     c3 = dataclasses.replace(c, etheta=2 * c.etheta)
     assert abs(ora.solve_all(c2)["cabs_sh"] / ref["cabs_sh"] - 4.0) < 1e-12
     assert abs(ora.solve_all(c3)["cabs_sh"] / ref["cabs_sh"] - 16.0) < 1e-11
+
+
+@pytest.mark.parametrize("eps", [4.0 + 0.3j, -20 + 1j])
+def test_fields_satisfy_interface_conditions(ora, eps):
+    """Result::getEHFields (Result.cpp:74-301) with the VSWFs of
+    AuxCoefficients (AuxCoefficients.cpp:30-214): for one sphere at the
+    origin the total field outside (incident + scattered) and the internal
+    field inside must have continuous tangential E and H and continuous
+    normal D = eps E and B = mu H across r = a, in every direction (the
+    fields at r = a (1 +- 1e-10)); and the incident expansion alone must
+    reproduce the plane wave E0 exp(i k.r) near the origin (to the accuracy of
+    the reference's 1e-6 shift of the polar angle and the truncation)."""
+    from optimet_amd.problem import Cluster
+    a = 80e-9
+    c = Cluster(xyz=[[0, 0, 0]], radius=[a], eps_r=[eps], eps_r_sh=[eps], nmax=8, wavelength=600e-9, theta_k=0.4,
+                phi_k=0.3, etheta=0.6, ephi=0.8j)
+    sol = ora.solve_all(c)
+    tb = ora.cg_tables(8)
+    rng = np.random.default_rng(0)
+    d = rng.standard_normal((20, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    fo, fi = ora.fields(c, tb, sol, d * a * (1 + 1e-10)), ora.fields(c, tb, sol, d * a * (1 - 1e-10))
+    tang = lambda v: v - np.sum(v * d, axis=1)[:, None] * d
+    for k in ("E", "H"):
+        assert np.max(np.abs(tang(fo[k]) - tang(fi[k]))) < 1e-8 * np.max(np.abs(tang(fo[k]))), k
+    nE_o, nE_i = np.sum(fo["E"] * d, axis=1), np.sum(fi["E"] * d, axis=1)
+    assert np.max(np.abs(nE_o - eps * nE_i)) < 1e-8 * np.max(np.abs(nE_o))
+    nH_o, nH_i = np.sum(fo["H"] * d, axis=1), np.sum(fi["H"] * d, axis=1)
+    assert np.max(np.abs(nH_o - nH_i)) < 1e-8 * np.max(np.abs(nH_o))
+    # the incident expansion about the origin against the plane wave
+    far = Cluster(xyz=[[5e-6, 0, 0]], radius=[10e-9], eps_r=[1.0001], eps_r_sh=[1.0], nmax=10, wavelength=600e-9,
+                  theta_k=0.4, phi_k=0.3, etheta=0.6, ephi=0.8j)
+    pts = d * 120e-9
+    f0 = ora.fields(far, ora.cg_tables(10), ora.solve_all(far), pts)
+    k = 2 * np.pi / 600e-9
+    th, ph = 0.4, 0.3
+    kv = k * np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+    eth = np.array([np.cos(th) * np.cos(ph), np.cos(th) * np.sin(ph), -np.sin(th)])
+    eph = np.array([-np.sin(ph), np.cos(ph), 0.0])
+    pw = (0.6 * eth + 0.8j * eph)[None, :] * np.exp(1j * pts @ kv)[:, None]
+    assert np.max(np.abs(f0["E"] - pw)) < 1e-7 * np.max(np.abs(pw))
