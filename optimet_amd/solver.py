@@ -234,19 +234,24 @@ class Solver:
             out.update({"x_sca_sh": xss, "x_int_sh": xis})
         return out
 
-    def sweep(self, wavelengths, batch=0, outputs=("x_sca", "x_int", "x_sca_sh", "x_int_sh")):
+    def sweep(self, wavelengths, batch=0, outputs=("x_sca", "x_int", "x_sca_sh", "x_int_sh"), eps=None,
+              eps_sh=None):
         """Batched wavelength sweep (omx_solve_sweep): the cluster at every
-        wavelength, Drude spheres re-evaluated per wavelength.  Returns a dict
-        with the requested (n_lambda, N) coefficient arrays, "xs" (n_lambda, 5:
-        C_ext, C_sca, C_abs, C_sca_SH, C_abs_SH) and "iterations" (n_lambda, 2)."""
+        wavelength, Drude spheres re-evaluated per wavelength (or the given
+        (n_lambda, P) permittivities eps, eps_sh).  Returns a dict with the
+        requested (n_lambda, N) coefficient arrays, "xs" (n_lambda, 5: C_ext,
+        C_sca, C_abs, C_sca_SH, C_abs_SH) and "iterations" (n_lambda, 2)."""
         c = self.cluster
         lams = np.ascontiguousarray(np.asarray(wavelengths, dtype=np.float64))
         nl, N = lams.size, c.N
-        eps = np.empty((nl, c.nobj), np.complex128)
-        eps_sh = np.empty((nl, c.nobj), np.complex128)
-        for i, lam in enumerate(lams):
-            ci = c.with_wavelength(lam)
-            eps[i], eps_sh[i] = ci.eps_r, ci.eps_r_sh
+        if eps is None:
+            eps = np.empty((nl, c.nobj), np.complex128)
+            eps_sh = np.empty((nl, c.nobj), np.complex128)
+            for i, lam in enumerate(lams):
+                ci = c.with_wavelength(lam)
+                eps[i], eps_sh[i] = ci.eps_r, ci.eps_r_sh
+        eps = np.ascontiguousarray(np.asarray(eps, np.complex128).reshape(nl, c.nobj))
+        eps_sh = np.ascontiguousarray(np.asarray(eps_sh, np.complex128).reshape(nl, c.nobj))
         out = {}
         ptrs = []
         for name in ("x_sca", "x_int", "x_sca_sh", "x_int_sh"):

@@ -1,0 +1,92 @@
+"""Case files end to end on the device: the driver reads a case in the
+reference's XML format, solves it through the C-ABI and writes the reference's
+output files; every number is checked against the oracle at the same inputs.
+Scans: C_ext, C_sca, C_abs, C_sca_SH, C_abs_SH per wavelength to 1e-9
+relative (the .dat files to their six printed digits).  Field maps: the
+fields against the oracle's fields at the device's own coefficients to 1e-10
+of each field's largest component, and against a full oracle solve to 1e-8."""
+import os
+
+import numpy as np
+import pytest
+
+from optimet_amd import driver
+
+pytestmark = pytest.mark.gpu
+
+CASES = os.path.join(os.path.dirname(__file__), "golden", "cases")
+KEYS = ("cext", "csca", "cabs", "csca_sh", "cabs_sh")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from optimet_amd import solver
+    return solver.Context()
+
+
+@pytest.fixture(scope="module")
+def ora():
+    import oracle
+    return oracle
+
+
+def read_dat(path):
+    return np.loadtxt(path, delimiter="\t", ndmin=2)
+
+
+@pytest.mark.parametrize("name", ["three_si_scan", "gold_pair_scan", "cube_lattice_scan"])
+def test_scan_case(ctx, ora, tmp_path, name):
+    case = driver.read_case(os.path.join(CASES, name + ".xml"))
+    print(f"case {name}: {case.cluster.nobj} spheres, nmax {case.nmax}, {len(case.wavelengths)} wavelengths",
+          flush=True)
+    prefix = str(tmp_path / name)
+    paths, xs = driver.run_case(case, prefix, kind="lu32ir", context=ctx)
+    nk = 5 if case.sh else 3
+    for i, lam in enumerate(case.wavelengths):
+        ref = ora.solve_all(case.at(lam))
+        for k in range(nk):
+            assert xs[i, k] == pytest.approx(ref[KEYS[k]], rel=1e-9), (name, lam, KEYS[k])
+    expect = {"_AbsorptionCS_FF.dat": xs[:, 0] - xs[:, 1], "_ScatteringCS_FF.dat": xs[:, 1]}
+    if case.sh:
+        expect.update({"_ScatteringCS_SH.dat": xs[:, 3], "_AbsorptionCS_SH.dat": xs[:, 4]})
+    assert sorted(paths) == sorted(prefix + s for s in expect)
+    for suffix, vals in expect.items():
+        d = read_dat(prefix + suffix)
+        np.testing.assert_allclose(d[:, 0], case.wavelengths, rtol=1e-5)
+        np.testing.assert_allclose(d[:, 1], vals, rtol=1e-5)
+
+
+@pytest.mark.parametrize("name", ["two_spheres_field", "two_spheres_field_projected"])
+def test_field_case(ctx, ora, tmp_path, name):
+    from optimet_amd import solver
+
+    case = driver.read_case(os.path.join(CASES, name + ".xml"))
+    print(f"case {name}: grid {case.grid}", flush=True)
+    prefix = str(tmp_path / name)
+    paths, f = driver.run_case(case, prefix, kind="lu32ir", context=ctx)
+    assert paths == [prefix + "_FF.npz", prefix + "_SH.npz"]
+    c = case.cluster
+    pts, shape = driver.grid_points(case.grid)
+    s = solver.Solver(c, kind="lu32ir", context=ctx)
+    s.update()
+    sol = s.solve()
+    tables = ora.cg_tables(c.nmax)
+    at_dev = ora.fields(c, tables, sol, pts)
+    full = ora.fields(c, tables, ora.solve_all(c), pts)
+    for key in ("E", "H", "E_sh", "H_sh"):
+        for ref, tol in ((at_dev, 1e-10), (full, 1e-8)):
+            r = ref[key]
+            if case.projection:
+                r = driver.spherical_projection(pts, c.xyz[0], r) if key in ("E", "H") else 0 * r
+            scale = np.max(np.abs(ref[key]))
+            assert np.max(np.abs(f[key] - r)) / scale < tol, (name, key, tol)
+    for fname, pair in ((paths[0], ("E", "H")), (paths[1], ("E_sh", "H_sh"))):
+        z = np.load(fname)
+        for group, key in zip(("Field_E", "Field_H"), pair):
+            for comp, ax in enumerate("XYZ"):
+                v = f[key][:, comp].reshape(shape[::-1]).transpose(2, 1, 0)
+                assert z[f"{group}/{ax}/real"].shape == shape
+                np.testing.assert_array_equal(z[f"{group}/{ax}/real"], v.real)
+                np.testing.assert_array_equal(z[f"{group}/{ax}/imag"], v.imag)
+            ab = np.sqrt(np.sum(np.abs(f[key]) ** 2, axis=1)).reshape(shape[::-1]).transpose(2, 1, 0)
+            np.testing.assert_allclose(z[f"{group}/ABS/abs"], ab, rtol=1e-15)
