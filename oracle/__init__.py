@@ -86,6 +86,9 @@ def _declare(L):
     L.ora_absorption_sh.argtypes = [_P, _D, _D, _D, _D]
     L.ora_fields.argtypes = [_P, _D, _D, _D, _D, _D, _I, _D, _D, _D, _D, _D]
     L.ora_time_sample.argtypes = [_P, _I, _I, _I, _D]
+    L.ora_aca_block.argtypes = [_I, _D, _Dbl, ctypes.POINTER(ctypes.c_int), _D, _D]
+    L.ora_aca_operator.argtypes = [_P, _I, _Dbl, _D, ctypes.POINTER(ctypes.c_int)]
+    L.ora_gmres_zcomp.argtypes = [_I, _D, _D, _Dbl, _I, _I, _D, _D]
 
 
 def _ptr(a):
@@ -252,6 +255,41 @@ def dense_solve(S, b, method=0):
     _check(lib().ora_dense_solve(N, _ptr(flat.view(np.float64)), _ptr(bb.view(np.float64)),
                                  _ptr(x.view(np.float64)), method))
     return x
+
+
+def aca_block(A, eps=1e-3):
+    """ACA_compression of the square block A: (U (n, r), V (r, n))."""
+    n = A.shape[0]
+    flat = np.asfortranarray(A, dtype=np.complex128).reshape(-1, order="F").copy()
+    U, V = _cz(n * n), _cz(n * n)
+    r = ctypes.c_int(0)
+    _check(lib().ora_aca_block(n, _ptr(flat.view(np.float64)), eps, ctypes.byref(r), _ptr(U.view(np.float64)),
+                               _ptr(V.view(np.float64))))
+    k = r.value
+    return U[:n * k].reshape(k, n).T.copy(), V[:n * k].reshape(k, n).copy()
+
+
+def aca_operator(prob, sh=False, eps=1e-3):
+    """The compressed operator (N, N) and the (P, P) ranks (-1 dense, 0 diagonal)."""
+    N, P = prob.N, prob.nobj
+    S = _cz(N * N)
+    rank = np.zeros(P * P, np.int32)
+    p = _Pinned(prob)
+    _check(lib().ora_aca_operator(p.ref(), int(sh), eps, _ptr(S.view(np.float64)),
+                                  rank.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    return S.reshape(N, N).T, rank.reshape(P, P)
+
+
+def gmres_zcomp(A, b, tol=1e-6, maxit=240, restarts=2):
+    """Gmres_Zcomp from x = 0: (x, steps, final residual estimate)."""
+    N = A.shape[0]
+    flat = np.asfortranarray(A, dtype=np.complex128).reshape(-1, order="F").copy()
+    bb = np.ascontiguousarray(b, dtype=np.complex128)
+    x = _cz(N)
+    info = np.zeros(2)
+    _check(lib().ora_gmres_zcomp(N, _ptr(flat.view(np.float64)), _ptr(bb.view(np.float64)), tol, maxit, restarts,
+                                 _ptr(x.view(np.float64)), _ptr(info)))
+    return x, int(info[0]), float(info[1])
 
 
 def cg_tables(nmax, nmaxs=None):

@@ -771,6 +771,236 @@ void dense_solve(int N, const std::vector<cd> &S, const std::vector<cd> &b, std:
     qr_solve(N, S, b.data(), x.data());
 }
 
+// ---------------------------------------------------------------- ACA
+// getMaxInd (PreconditionedMatrix.cpp:864-889): the first index of largest
+// modulus among the entries whose index is not one of used[0 .. nused-1];
+// -1 when no such entry is non-zero (the reference then returns an unset
+// index).
+int max_index(const cd *v, int n, const std::vector<int> &used, int nused) {
+  double mx = 0.0;
+  int im = -1;
+  for(int i = 0; i < n; ++i) {
+    bool same = false;
+    for(int j = 0; j < nused; ++j)
+      if(used[j] == i) same = true;
+    if(same) continue;
+    double a = std::abs(v[i]);
+    if(a > mx) {
+      mx = a;
+      im = i;
+    }
+  }
+  return im;
+}
+
+// ACA_compression (PreconditionedMatrix.cpp:763-862): partially pivoted
+// adaptive cross approximation of the n x n block A (column-major),
+// A ~ U V, U n x r (column p at U + p n), V r x n (row p at V + p n).
+// Step k: residual row I_k, its largest entry J_k (columns not yet used),
+// v_k = row / pivot, residual column J_k -> u_k; the squared Frobenius
+// estimate grows by |u_k|^2 |v_k|^2 + 2 sum_{p = 0}^{k-2}
+// |sum_{t < k} u_p[t] u_k[t]| |sum_{t < k} v_p[t] v_k[t]| (the reference's
+// loop bounds and unconjugated products, :826-843); the iteration stops
+// after step k >= 1 when eps sqrt(estimate) >= |u_k| |v_k|, else I_{k+1} is
+// the largest entry of the residual column among the unused rows.  When a
+// residual row or column has no non-zero candidate left the k terms
+// reproduce A exactly and the iteration stops (the reference reads an unset
+// index there).  Returns r.
+int aca_block(int n, const cd *A, cd *U, cd *V, double eps) {
+  std::vector<int> I(1, 0), J;
+  std::vector<cd> row(n);
+  auto nrm = [n](const cd *v) {
+    double s = 0;
+    for(int i = 0; i < n; ++i) s += std::norm(v[i]);
+    return std::sqrt(s);
+  };
+  double norma = 0.0;
+  int r = 0;
+  for(int k = 0; k < n; ++k) {
+    const int ik = I[k];
+    for(int c = 0; c < n; ++c) {
+      cd s = 0.0;
+      for(int p = 0; p < k; ++p) s += U[(size_t)p * n + ik] * V[(size_t)p * n + c];
+      row[c] = A[(size_t)c * n + ik] - s;
+    }
+    const int jk = max_index(row.data(), n, J, k);
+    if(jk < 0) break;
+    J.push_back(jk);
+    const cd piv = row[jk];
+    cd *vk = V + (size_t)k * n, *uk = U + (size_t)k * n;
+    for(int c = 0; c < n; ++c) vk[c] = row[c] / piv;
+    for(int q = 0; q < n; ++q) {
+      cd s = 0.0;
+      for(int p = 0; p < k; ++p) s += V[(size_t)p * n + jk] * U[(size_t)p * n + q];
+      uk[q] = A[(size_t)jk * n + q] - s;
+    }
+    r = k + 1;
+    const double cn = nrm(uk), rn = nrm(vk);
+    if(k == 0) {
+      norma = cn * cn * (rn * rn);
+    } else {
+      double sum = 0.0;
+      for(int p = 0; p != k - 1; ++p) {
+        cd p1 = 0.0, p2 = 0.0;
+        for(int t = 0; t != k; ++t) {
+          p1 += U[(size_t)p * n + t] * uk[t];
+          p2 += V[(size_t)p * n + t] * vk[t];
+        }
+        sum += std::abs(p1) * std::abs(p2);
+      }
+      norma = norma + cn * cn * (rn * rn) + 2.0 * sum;
+      if(eps * std::sqrt(norma) >= cn * rn) break;
+    }
+    const int in = max_index(uk, n, I, k + 1);
+    if(in < 0) break;
+    I.push_back(in);
+  }
+  return r;
+}
+
+// Admissibility of the pair (i, j) for compression: i != j and
+// |c_i - c_j| >= 2 (r_i + r_j) (PreconditionedMatrix.cpp:526, the distance of
+// Tools::findDistance, Tools.cpp:30-38).
+bool admissible(const Problem &pb, int i, int j) {
+  if(i == j) return false;
+  Cart a = to_cartesian(pb.obj[i].vR), b = to_cartesian(pb.obj[j].vR);
+  double d = std::sqrt(std::pow(b.x - a.x, 2.0) + std::pow(b.y - a.y, 2.0) + std::pow(b.z - a.z, 2.0));
+  return d >= 2.0 * (pb.obj[i].radius + pb.obj[j].radius);
+}
+
+// Scattering_matrix_ACA_FF / _SH (PreconditionedMatrix.cpp:489-551,
+// 696-759): every admissible block compressed, the others dense, expanded
+// back to the N x N compressed operator (column-major) for the dense
+// solvers; rank[i P + j] = r of a compressed block, -1 for a dense block,
+// 0 on the diagonal.
+void aca_operator(const Problem &pb, bool sh, double eps, std::vector<cd> &S, int *rank) {
+  const int nf = pb.nf, P = pb.P, n2 = 2 * nf;
+  const size_t N = (size_t)n2 * P;
+  S.assign(N * N, cd(0, 0));
+  parallel_for(P, [&](int i) {
+    auto T = object_vector(pb, pb.obj[i], sh ? 1 : 0, pb.bg_eps, pb.bg_mu);
+    std::vector<cd> blk((size_t)n2 * n2), U((size_t)n2 * n2), V((size_t)n2 * n2);
+    for(int j = 0; j < P; ++j) {
+      cd *dst = &S[(size_t)j * n2 * N + (size_t)i * n2];
+      if(!admissible(pb, i, j)) {
+        scattering_block(pb, sh, i, j, T, dst, N);
+        if(rank) rank[(size_t)i * P + j] = i == j ? 0 : -1;
+        continue;
+      }
+      scattering_block(pb, sh, i, j, T, blk.data(), n2);
+      int r = aca_block(n2, blk.data(), U.data(), V.data(), eps);
+      if(rank) rank[(size_t)i * P + j] = r;
+      for(int c = 0; c < n2; ++c)
+        for(int q = 0; q < n2; ++q) {
+          cd s = 0.0;
+          for(int p = 0; p < r; ++p) s += U[(size_t)p * n2 + q] * V[(size_t)p * n2 + c];
+          dst[(size_t)c * N + q] = s;
+        }
+    }
+  });
+}
+
+// Gmres_Zcomp (PreconditionedMatrix.cpp:892-985) with det_approx
+// (:1087-1130) on the operator A (N x N column-major): restarted GMRES from
+// x = 0, modified Gram-Schmidt, the Hessenberg column rotated by the
+// reference's Givens rotations (c, s from the ratio of the smaller to the
+// larger entry, W = [[conj c, -conj s], [s, c]]), convergence when
+// |g_{n+1}| / |b| <= tol, at most maxit steps per cycle and `restarts`
+// cycles; the small triangular system is solved by back substitution (the
+// reference's column-pivoted QR of the same triangle).  info: steps,
+// final estimate.
+void gmres_zcomp(int N, const std::vector<cd> &A, const cd *b, double tol, int maxit, int restarts, cd *xout,
+                 double *info) {
+  auto mv = [&](const std::vector<cd> &x, std::vector<cd> &y) {
+    y.assign(N, 0.0);
+    for(int c = 0; c < N; ++c) {
+      const cd xc = x[c];
+      const cd *col = &A[(size_t)c * N];
+      for(int i = 0; i < N; ++i) y[i] += col[i] * xc;
+    }
+  };
+  auto norm = [N](const std::vector<cd> &v) {
+    double s = 0;
+    for(int i = 0; i < N; ++i) s += std::norm(v[i]);
+    return std::sqrt(s);
+  };
+  std::vector<cd> x(N, 0.0), w, res(N);
+  std::vector<std::vector<cd>> v;
+  std::vector<cd> H, Ri, gi;
+  double abs_y = 0;
+  for(int i = 0; i < N; ++i) abs_y += std::norm(b[i]);
+  abs_y = std::sqrt(abs_y);
+  std::vector<double> err(1, 1.0);
+  int n = 0, steps = 0;
+  for(int rest = 1; rest <= restarts; ++rest) {
+    if(err[n] <= tol) break;
+    mv(x, w);
+    for(int i = 0; i < N; ++i) res[i] = b[i] - w[i];
+    const double beta = norm(res);
+    v.assign(1, std::vector<cd>(N));
+    for(int i = 0; i < N; ++i) v[0][i] = res[i] / beta;
+    // Ri: the rotated Hessenberg matrix, (maxit + 1) x maxit column-major; gi the rotated beta e_1
+    const int ld = maxit + 1;
+    Ri.assign((size_t)ld * maxit, 0.0);
+    gi.assign(ld, 0.0);
+    gi[0] = beta;
+    std::vector<cd> cs, sn;
+    n = 0;
+    err.resize(1);  // err(0) = 1 throughout (:902); the later entries are rewritten
+    while(n < maxit && err[n] > tol) {
+      mv(v[n], w);
+      cd *h = &Ri[(size_t)n * ld];
+      for(int t = 0; t <= n; ++t) {
+        cd d = 0.0;
+        for(int i = 0; i < N; ++i) d += std::conj(v[t][i]) * w[i];
+        h[t] = d;
+        for(int i = 0; i < N; ++i) w[i] -= d * v[t][i];
+      }
+      h[n + 1] = norm(w);
+      v.emplace_back(N);
+      for(int i = 0; i < N; ++i) v[n + 1][i] = w[i] / h[n + 1];
+      for(int i = 0; i < n; ++i) {  // the earlier rotations on the new column
+        cd a = h[i], bb = h[i + 1];
+        h[i] = std::conj(cs[i]) * a + std::conj(-sn[i]) * bb;
+        h[i + 1] = sn[i] * a + cs[i] * bb;
+      }
+      cd hi1 = h[n], hi2 = h[n + 1], c, s;
+      if(std::abs(hi2) > std::abs(hi1)) {
+        cd temp = hi1 / hi2;
+        s = 1.0 / std::sqrt(1.0 + std::pow(std::abs(temp), 2));
+        c = -temp * s;
+      } else {
+        cd temp = hi2 / hi1;
+        c = 1.0 / std::sqrt(1.0 + std::pow(std::abs(temp), 2));
+        s = -temp * c;
+      }
+      cs.push_back(c);
+      sn.push_back(s);
+      h[n] = std::conj(c) * hi1 + std::conj(-s) * hi2;
+      h[n + 1] = s * hi1 + c * hi2;
+      cd g0 = gi[n], g1 = gi[n + 1];
+      gi[n] = std::conj(c) * g0 + std::conj(-s) * g1;
+      gi[n + 1] = s * g0 + c * g1;
+      err.push_back(std::abs(gi[n + 1]) / abs_y);
+      ++n;
+      ++steps;
+    }
+    std::vector<cd> y(n);
+    for(int i = n - 1; i >= 0; --i) {
+      cd s = gi[i];
+      for(int k = i + 1; k < n; ++k) s -= Ri[(size_t)k * ld + i] * y[k];
+      y[i] = s / Ri[(size_t)i * ld + i];
+    }
+    for(int j = 0; j < n; ++j)
+      for(int i = 0; i < N; ++i) x[i] += y[j] * v[j][i];
+  }
+  std::copy(x.begin(), x.end(), xout);
+  if(info) {
+    info[0] = steps;
+    info[1] = err[n];
+  }
+}
+
 // ---------------------------------------------------------------- Wigner
 long double fact(int n) {
   static const std::vector<long double> t = [] {  // initialised once, thread-safe
@@ -1444,6 +1674,31 @@ int ora_dense_solve(int N, const double *S, const double *b, double *x, int meth
   });
 }
 
+int ora_aca_block(int n, const double *A, double eps, int *rank, double *U, double *V) {
+  return guard([&] {
+    if(n < 1) throw OraError("block order must be positive");
+    *rank = aca_block(n, reinterpret_cast<const cd *>(A), reinterpret_cast<cd *>(U), reinterpret_cast<cd *>(V), eps);
+  });
+}
+
+int ora_aca_operator(const ora_problem *in, int sh, double eps, double *S, int *rank) {
+  return guard([&] {
+    Problem pb = make_problem(in);
+    std::vector<cd> s;
+    aca_operator(pb, sh != 0, eps, s, rank);
+    std::memcpy(S, s.data(), s.size() * sizeof(cd));
+  });
+}
+
+int ora_gmres_zcomp(int N, const double *A, const double *b, double tol, int maxit, int restarts, double *x,
+                    double *info) {
+  return guard([&] {
+    if(N < 1 || maxit < 1 || restarts < 1) throw OraError("invalid GMRES settings");
+    std::vector<cd> a(reinterpret_cast<const cd *>(A), reinterpret_cast<const cd *>(A) + (size_t)N * N);
+    gmres_zcomp(N, a, reinterpret_cast<const cd *>(b), tol, maxit, restarts, reinterpret_cast<cd *>(x), info);
+  });
+}
+
 int ora_cg_tables(int nmax, int nmaxs, double *tables) {
   return guard([&] { cg_tables(nmax, nmaxs, tables); });
 }
@@ -1475,9 +1730,21 @@ int ora_solve_all(const ora_problem *in, int method, double *x_sca, double *x_in
     // update(): PreconditionedMatrixSolver.h:82-100
     auto Q = source_vector(pb);
     std::vector<cd> S, X;
-    scattering_matrix(pb, false, S);
+    // method 2: the ACA-compressed operators and Gmres_Zcomp with the
+    // reference's settings tol 1e-6, 240 steps, 2 cycles
+    // (PreconditionedMatrixSolver.h:47-56, 84-96)
+    auto solve_sys = [&](bool sh, const std::vector<cd> &rhs, std::vector<cd> &out) {
+      if(method == 2) {
+        aca_operator(pb, sh, 1e-3, S, nullptr);
+        out.assign(N, 0.0);
+        gmres_zcomp((int)N, S, rhs.data(), 1e-6, 240, 2, out.data(), nullptr);
+      } else {
+        scattering_matrix(pb, sh, S);
+        dense_solve((int)N, S, rhs, out, method);
+      }
+    };
     // solve(): PreconditionedMatrixSolver.h:45-79
-    dense_solve((int)N, S, Q, X, method);
+    solve_sys(false, Q, X);
     std::vector<cd> XI(N);
     for(int j = 0; j < pb.P; ++j) {  // convertInternal (Solver.cpp:57-77)
       auto ia = object_vector(pb, pb.obj[j], 2, pb.bg_eps, pb.bg_mu);
@@ -1489,10 +1756,9 @@ int ora_solve_all(const ora_problem *in, int method, double *x_sca, double *x_in
     if(pb.sh) {
       std::vector<double> tab((size_t)9 * nf * nf * nf);
       cg_tables(pb.nmax, pb.nmax, tab.data());
-      std::vector<cd> K, K1, V;
+      std::vector<cd> K, K1;
       source_sh(pb, tab.data(), XI, K, K1);
-      scattering_matrix(pb, true, V);
-      dense_solve((int)N, V, K, XS, method);
+      solve_sys(true, K, XS);
       XIS.resize(N);
       for(int j = 0; j < pb.P; ++j) {  // convertInternal_SH (Solver.cpp:95-116)
         auto ia = object_vector(pb, pb.obj[j], 3, pb.bg_eps, pb.bg_mu);

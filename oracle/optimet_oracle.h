@@ -106,7 +106,9 @@ int ora_source_sh(const ora_problem *pb, const double *tables, const double *x_i
                   double *K1ana);
 
 /* Full PreconditionedMatrix update()+solve(): x_sca, x_int (N complex each) and,
- * if pb->sh, x_sca_sh, x_int_sh.  xs: [Cext, Csca, Cabs, Csca_SH, Cabs_SH]. */
+ * if pb->sh, x_sca_sh, x_int_sh.  xs: [Cext, Csca, Cabs, Csca_SH, Cabs_SH].
+ * method 0: LU; 1: column-pivoted QR; 2: ACA-compressed operators + Gmres_Zcomp
+ * (tol 1e-6, 240 steps, 2 cycles). */
 int ora_solve_all(const ora_problem *pb, int method, double *x_sca, double *x_int,
                   double *x_sca_sh, double *x_int_sh, double *xs);
 
@@ -125,6 +127,23 @@ int ora_absorption_sh(const ora_problem *pb, const double *tables, const double 
 int ora_fields(const ora_problem *pb, const double *tables, const double *x_sca, const double *x_int,
                const double *x_sca_sh, const double *x_int_sh, int npts, const double *xyz, double *E, double *H,
                double *Esh, double *Hsh);
+
+/* ACA_compression of the n x n block A (column-major) at tolerance eps:
+ * A ~ U V with U n x r (column-major, n x n capacity) and V r x n (row-major,
+ * n x n capacity); *rank = r. */
+int ora_aca_block(int n, const double *A, double eps, int *rank, double *U, double *V);
+
+/* Scattering_matrix_ACA_FF (sh=0) / _SH (sh=1) expanded to the N x N
+ * compressed operator S (column-major): admissible blocks (i != j,
+ * |c_i - c_j| >= 2 (r_i + r_j)) replaced by U V.  rank: P*P entries, r of a
+ * compressed block, -1 for a dense block, 0 on the diagonal (may be null). */
+int ora_aca_operator(const ora_problem *pb, int sh, double eps, double *S, int *rank);
+
+/* Gmres_Zcomp on the N x N operator A (column-major) from x = 0: tol on the
+ * rotated residual estimate, maxit steps per cycle, `restarts` cycles.
+ * info[0] = steps taken, info[1] = final relative residual estimate. */
+int ora_gmres_zcomp(int N, const double *A, const double *b, double tol, int maxit, int restarts, double *x,
+                    double *info);
 
 /* CPU-baseline timing sample: out[0] s per pair coupling + block, out[1] s
  * of one dense solve of order nsolve (method as ora_dense_solve), out[2] s of

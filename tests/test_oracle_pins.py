@@ -625,3 +625,99 @@ def test_fields_satisfy_interface_conditions(ora, eps):
     eph = np.array([-np.sin(ph), np.cos(ph), 0.0])
     pw = (0.6 * eth + 0.8j * eph)[None, :] * np.exp(1j * pts @ kv)[:, None]
     assert np.max(np.abs(f0["E"] - pw)) < 1e-7 * np.max(np.abs(pw))
+
+
+# ---------------------------------------------------------------- ACA
+# ACA_compression / getMaxInd (PreconditionedMatrix.cpp:763-889) and
+# Gmres_Zcomp / det_approx (:892-985, 1087-1130).
+
+def test_aca_known_answer_with_the_reference_norm_estimate(ora):
+    """A 4 x 4 worked by hand.  A = [[1,2,4,0],[3,1,1,0],[3,1,8,0],[.5,0,0,2]]:
+    k=0: row 0, pivot column 2, v0 = (1/4, 1/2, 1, 0), u0 = (4, 1, 8, 0);
+    largest |u0| off row 0 -> row 2.  k=1: residual row 2 = (1, -3, 0, 0), pivot
+    column 1, v1 = (-1/3, 1, 0, 0), u1 = (0, 1/2, -3, 0); estimate
+    81 * 21/16 + 37/4 * 10/9 = 116.5903; next row 1.  k=2: residual row 1 =
+    (35/12, 0, 0, 0), pivot column 0, v2 = e0, u2 = (0, 35/12, 0, 1/2).  The
+    reference's estimate adds 2 sum_{p <= k-2} |sum_{t<k} u_p[t] u_k[t]|
+    |sum_{t<k} v_p[t] v_k[t]| = 2 (35/12)(1/4): 126.8056, so at eps = 0.2623
+    eps sqrt(126.8056) = 2.9537 < |u2||v2| = 2.9592 and a fourth term follows
+    (u3 = 2 e3, v3 = e3).  The textbook sum over p <= k-1 and all t adds
+    (35/24)(1/3) more (127.7778 -> 2.9650 >= 2.9592) and stops at rank 3."""
+    A = np.array([[1, 2, 4, 0], [3, 1, 1, 0], [3, 1, 8, 0], [0.5, 0, 0, 2]], dtype=complex)
+    U, V = ora.aca_block(A, eps=0.2623)
+    Ue = np.array([[4, 0, 0, 0], [1, 0.5, 35 / 12, 0], [8, -3, 0, 0], [0, 0, 0.5, 2]])
+    Ve = np.array([[0.25, 0.5, 1, 0], [-1 / 3, 1, 0, 0], [1, 0, 0, 0], [0, 0, 0, 1]])
+    np.testing.assert_allclose(U, Ue, atol=1e-14)
+    np.testing.assert_allclose(V, Ve, atol=1e-14)
+    np.testing.assert_allclose(U @ V, A, atol=1e-14)
+    U, V = ora.aca_block(A, eps=0.2633)  # above 2.9592 / sqrt(126.8056) = 0.26279: stops after k = 2
+    assert U.shape == (4, 3)
+
+
+def test_aca_first_maximum_wins(ora):
+    """getMaxInd keeps the first of equal moduli: row (1, -1) pivots on column
+    0, giving U = [[1, 0], [1, 2]], V = [[1, -1], [0, 1]]."""
+    U, V = ora.aca_block(np.array([[1, -1], [1, 1]], dtype=complex))
+    np.testing.assert_allclose(U, [[1, 0], [1, 2]], atol=1e-15)
+    np.testing.assert_allclose(V, [[1, -1], [0, 1]], atol=1e-15)
+
+
+@pytest.mark.parametrize("r", [1, 2, 5])
+def test_aca_recovers_low_rank_matrices(ora, r):
+    """An exactly rank-r matrix is reproduced to rounding after r terms (the
+    (r+1)-th residual is at rounding level and the estimate stops there)."""
+    rng = np.random.default_rng(r)
+    n = 24
+    A = (rng.normal(size=(n, r)) + 1j * rng.normal(size=(n, r))) @ (rng.normal(size=(r, n)) + 1j * rng.normal(size=(r, n)))
+    U, V = ora.aca_block(A)
+    assert r <= U.shape[1] <= r + 1
+    assert np.linalg.norm(U @ V - A) <= 1e-12 * np.linalg.norm(A)
+
+
+def test_aca_coupling_blocks(ora):
+    """On the admissible coupling blocks of a cluster the compression error
+    stays near eps (median below 1e-2 relative), every reported rank is the
+    numerical rank of the block's factors, non-admissible blocks and the
+    diagonal are exact, and admissibility is |c_i - c_j| >= 2 (r_i + r_j)."""
+    from optimet_amd.problem import drude_cluster
+    c = drude_cluster(10, 4, 800e-9, 30, 60, 600, 10, seed=7, sh=False)
+    Sc, rank = ora.aca_operator(c)
+    S = ora.scattering_matrix(c)
+    n2 = 2 * c.nf
+    d = np.linalg.norm(c.xyz[:, None, :] - c.xyz[None, :, :], axis=2)
+    adm = (d >= 2 * (c.radius[:, None] + c.radius[None, :])) & ~np.eye(c.nobj, dtype=bool)
+    assert np.array_equal(rank > 0, adm) and np.all(np.diag(rank) == 0) and np.all(rank[~adm & ~np.eye(c.nobj, dtype=bool)] == -1)
+    assert 0 < adm.sum() < c.nobj * (c.nobj - 1)  # both kinds present
+    errs = []
+    for i in range(c.nobj):
+        for j in range(c.nobj):
+            b = (slice(i * n2, (i + 1) * n2), slice(j * n2, (j + 1) * n2))
+            if rank[i, j] <= 0:
+                np.testing.assert_array_equal(Sc[b], S[b])
+            else:
+                errs.append(np.linalg.norm(Sc[b] - S[b]) / np.linalg.norm(S[b]))
+                sv = np.linalg.svd(Sc[b], compute_uv=False)
+                assert np.sum(sv > 1e-10 * sv[0]) == rank[i, j]
+    assert np.median(errs) < 1e-2 and max(errs) < 0.2
+
+
+def test_gmres_zcomp_residual_and_termination(ora):
+    """The rotated estimate |g_{n+1}| / |b| equals the true residual (a wrong
+    rotation breaks that); GMRES on an operator with d distinct eigenvalues
+    ends in d steps; restarts continue from the last iterate."""
+    rng = np.random.default_rng(4)
+    n = 40
+    A = np.eye(n) * 3 + (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) / (4 * np.sqrt(n))
+    b = rng.normal(size=n) + 1j * rng.normal(size=n)
+    x, steps, est = ora.gmres_zcomp(A, b, tol=1e-10)
+    true = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    assert est <= 1e-10 and true == pytest.approx(est, rel=1e-3)
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-9)
+    Q, _ = np.linalg.qr(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    D = np.repeat([1.0, 2.0, 5.0 + 1j], [15, 15, 10])
+    x, steps, est = ora.gmres_zcomp(Q @ np.diag(D) @ Q.conj().T, b, tol=1e-12)
+    assert steps == 3
+    x, steps, est = ora.gmres_zcomp(A, b, tol=1e-13, maxit=4, restarts=3)
+    assert steps == 12 and est > 1e-13
+    x1, _, _ = ora.gmres_zcomp(A, b, tol=1e-13, maxit=4, restarts=1)
+    assert np.linalg.norm(b - A @ x) < np.linalg.norm(b - A @ x1)
