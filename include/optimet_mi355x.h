@@ -55,6 +55,11 @@ extern "C" {
 #define OMX_SOLVER_LU64 0     /* complex FP64 LU (+ FP64 refinement after block pivoting) */
 #define OMX_SOLVER_LU32_IR 1  /* complex FP32 LU on MFMA + FP64 GMRES-IR */
 #define OMX_SOLVER_GMRES64 2  /* restarted GMRES, FP64 matrix streamed from HBM */
+#define OMX_SOLVER_ACA_GMRES 3 /* <ACA compression="yes"/>: admissible blocks compressed by
+                                  ACA at 1e-3, restarted GMRES on the compressed operator
+                                  (PreconditionedMatrixSolver.h:47-56, 84-96; the
+                                  reference's settings are tol 1e-6, 240 steps per cycle,
+                                  2 cycles) */
 
 /* Pivot search of the LU factorisations (omx_set_pivoting). */
 #define OMX_PIVOT_BLOCK 0     /* among the rows of each 64 x 64 diagonal block, then
@@ -123,7 +128,7 @@ int omx_set_excitation(omx_context *ctx, double wavelength, double theta, double
 
 /* Solver selection; tol is the relative residual target of the iterative
  * parts (GMRES-IR refinement, GMRES), maxit their iteration cap, restart the
- * GMRES restart length. */
+ * GMRES restart length (<= 256). */
 int omx_set_solver(omx_context *ctx, int kind, double tol, int maxit, int restart);
 
 /* Pivot search of the LU solvers (default OMX_PIVOT_BLOCK).  The reference
@@ -190,6 +195,23 @@ int omx_read_vector(omx_context *ctx, int which, double *out);
  * rows row0 .. row0+nrows-1 and columns col0 .. col0+ncols-1, copied to out
  * as an nrows x ncols column-major complex128 array (no full-matrix copy). */
 int omx_read_matrix_block(omx_context *ctx, int which, int row0, int col0, int nrows, int ncols, double *out);
+
+/* The compressed operator of OMX_SOLVER_ACA_GMRES after omx_update, S (which
+ * 0) or V (which 1), as Scattering_matrix_ACA_FF / _SH build it
+ * (PreconditionedMatrix.cpp:489-551, 696-759).  ranks (P*P, may be NULL):
+ * r of a compressed block (i, j) at ranks[i P + j], -1 for a dense block, 0
+ * on the diagonal.  info (7 doubles, may be NULL): device bytes of the
+ * operator's blocks, bytes of the dense FP64 matrix (16 N^2), the size the
+ * reference reports in MB (16 bytes per stored entry, identity blocks
+ * included, :547-549), compressed blocks, dense off-diagonal blocks, largest
+ * and mean rank. */
+int omx_aca_info(omx_context *ctx, int which, int *ranks, double *info);
+
+/* Block (i, j) of the compressed operator: *rank = r with U (n x r,
+ * column-major) and V (r x n, row-major) written to U, V (n*n complex
+ * capacity each, n = 2 nmax (nmax + 2)); a dense block gives *rank = -1 and
+ * the block in U (n x n column-major); the diagonal gives *rank = 0. */
+int omx_aca_block(omx_context *ctx, int which, int i, int j, int *rank, double *U, double *V);
 
 /* Wavelength sweep, batched: the current geometry and excitation solved at
  * nl vacuum wavelengths (m), as Simulation::scan_wavelengths does one

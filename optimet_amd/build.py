@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liboptimet_mi355x.so")
-SOURCES = ["omx_assemble.hip", "omx_incident.hip", "omx_lu.hip", "omx_iter.hip", "omx_sh.hip", "omx_capi.hip"]
+SOURCES = ["omx_assemble.hip", "omx_incident.hip", "omx_lu.hip", "omx_iter.hip", "omx_sh.hip", "omx_aca.hip",
+           "omx_capi.hip"]
 # Per-file flags: the plane-wave coefficients reproduce the reference's
 # double arithmetic operation by operation (no fused multiply-adds).
 EXTRA = {"omx_incident.hip": ["-ffp-contract=off"]}

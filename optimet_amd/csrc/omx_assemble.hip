@@ -337,9 +337,20 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs a) {
     if(a.S64) a.S64 += z * a.zS;
     if(a.S32) a.S32 += z * a.zS;
   }
-  const int pid = blockIdx.x;
+  int pid = blockIdx.x;
+  size_t x0, y0;
+  if(a.plist) {  // block blockIdx.x of a pair list, in its own slot
+    pid = a.plist[blockIdx.x];
+    if(a.S64) a.S64 += blockIdx.x * a.zblk;
+    if(a.S32) a.S32 += blockIdx.x * a.zblk;
+  }
   const int i = pid / P, j = pid % P;
-  const size_t x0 = (size_t)i * 2 * NF, y0 = (size_t)j * 2 * NF;
+  if(a.plist) {
+    x0 = y0 = 0;
+  } else {
+    x0 = (size_t)i * 2 * NF;
+    y0 = (size_t)j * 2 * NF;
+  }
   const int t = threadIdx.x;
   const int cg = blockDim.x / NF;  // column groups
   const int q = t % NF, grp = t / NF;
@@ -528,7 +539,7 @@ __global__ void pad_kernel(T *A, int N, int Np, size_t lda, size_t zs) {
 // ------------------------------------------------------------------ launchers
 template <int NMAX>
 static void launch_pair(const PairArgs &a, hipStream_t st, int nb) {
-  hipLaunchKernelGGL(pair_kernel<NMAX>, dim3(a.P * a.P, 1, nb), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(pair_kernel<NMAX>, dim3(a.plist ? a.nlist : a.P * a.P, 1, nb), dim3(256), 0, st, a);
 }
 template <int NMAX>
 static void launch_origin(const OriginArgs &a, hipStream_t st, int nb) {

@@ -134,6 +134,28 @@ struct omx_context {
     DBuf perm, perm2, flags, flags2, dinv, dinv2, map, status, basis, basisZ, r, w1, w2, gpart, part, hd, nrm, sc,
         act, kz, omz;
   } sw;
+  // ACA-compressed operators S (0) and V (1) of OMX_SOLVER_ACA_GMRES
+  struct AcaMat {
+    int n = 0, P = 0, G = 0, naca = 0, ndense = 0, rmax = 0;
+    double rsum = 0, bytes = 0, ref_mb = 0;
+    std::vector<int> rank;           // [i P + j]: r, -1 dense, 0 diagonal
+    std::vector<const cplx *> where;  // [i P + j]: U (compressed) or the block (dense)
+    std::vector<DBuf> pools;          // factor pools, one per compression chunk
+    DBuf dense, desc, rowptr, acalist, t, part;
+    bool built = false;
+    void release(size_t *T) {
+      for(auto &p : pools) p.release(T);
+      pools.clear();
+      DBuf *b[] = {&dense, &desc, &rowptr, &acalist, &t, &part};
+      for(DBuf *x : b) x->release(T);
+      rank.clear();
+      where.clear();
+      built = false;
+      bytes = ref_mb = rsum = 0;
+      naca = ndense = rmax = 0;
+    }
+  } aca[2];
+  DBuf d_acaA, d_acaU, d_acaV, d_acaR, d_acaOff, d_plist;
   omx_stats stats{};
   void *scratch = nullptr;  // this device's address of the no-allocation solve's scratch array
   long long mv_calls = 0;  // FP64 matrix-vector products since the last omx_set_profile
@@ -165,6 +187,7 @@ template <typename F> int guard(F &&f) {
 cplx C(const double *p, int i) { return omx::mk(p[2 * i], p[2 * i + 1]); }
 
 bool use32(int kind) { return kind == OMX_SOLVER_LU32_IR; }
+bool is_lu(int kind) { return kind == OMX_SOLVER_LU64 || kind == OMX_SOLVER_LU32_IR; }
 
 void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags, DBuf &dinv, bool second) {
   const int Np = c->Np;
@@ -254,7 +277,9 @@ void alloc_all(omx_context *c) {
 void alloc_matrices(omx_context *c) {
   const size_t n2 = (size_t)c->Np * c->Np;
   size_t *T = &c->total;
-  bool need64 = true;  // FP64 matrix: factorised (LU64), streamed (GMRES) or the residual operator (IR)
+  // FP64 matrix: factorised (LU64), streamed (GMRES) or the residual operator
+  // (IR); the compressed operators of ACA replace it
+  bool need64 = c->kind != OMX_SOLVER_ACA_GMRES;
   bool need32 = use32(c->kind);
   if(need64) c->d_S64.ensure(sizeof(cplx) * n2, T); else c->d_S64.release(T);
   if(need32) c->d_S32.ensure(sizeof(float2) * n2, T); else c->d_S32.release(T);
@@ -269,7 +294,9 @@ void alloc_matrices(omx_context *c) {
   if(f64) c->d_F64.ensure(sizeof(cplx) * n2, T); else c->d_F64.release(T);
   if(f64 && c->sh) c->d_F64sh.ensure(sizeof(cplx) * n2, T); else c->d_F64sh.release(T);
   c->d_basis.ensure(sizeof(cplx) * (size_t)c->Np * (c->restart + 1), T);
-  c->d_basisZ.ensure(sizeof(cplx) * (size_t)c->Np * (c->restart + 1), T);
+  // preconditioned (flexible) GMRES keeps M^{-1} v_j: the LU kinds only
+  if(is_lu(c->kind)) c->d_basisZ.ensure(sizeof(cplx) * (size_t)c->Np * (c->restart + 1), T);
+  else c->d_basisZ.release(T);
   alloc_lu(c, c->lu, c->d_piv, c->d_perm, c->d_flags, c->d_dinv, false);
   if(c->sh) alloc_lu(c, c->lu_sh, c->d_piv2, c->d_perm2, c->d_flags2, c->d_dinv2, true);
 }
@@ -278,6 +305,170 @@ float elapsed(omx_context *c, int a, int b) {
   float ms = 0;
   hipEventElapsedTime(&ms, c->ev[a], c->ev[b]);
   return ms;
+}
+
+// ------------------------------------------------------------------ ACA
+// Scattering_matrix_ACA_FF / _SH (PreconditionedMatrix.cpp:489-551,
+// 696-759) on the device: block (i, j != i) is compressed when
+// |c_i - c_j| >= 2 (r_i + r_j) (:526; the distance of Tools::findDistance on
+// the reader's spherical coordinates, Tools.cpp:30-44) and kept dense
+// otherwise.  Dense blocks are assembled by the pair kernel straight into
+// their pool; compressed blocks are assembled chunk by chunk into n x n
+// staging slots, compressed there (aca_kernel, at most n terms each) and
+// their factors packed into a pool of exactly sum 2 n r entries per chunk,
+// so the full matrix never exists on the device.
+constexpr double kAcaEps = 1e-3;            // PreconditionedMatrix.cpp:774
+constexpr double kAcaStagingBytes = 1.5e9;  // assembly + factor staging of one chunk
+
+void aca_build(omx_context *c, bool shf) {
+  omx_context::AcaMat &m = c->aca[shf ? 1 : 0];
+  size_t *T = &c->total;
+  m.release(T);
+  const int P = c->P, nmax = c->nmax, n = 2 * c->nf;
+  const size_t n2 = (size_t)n * n;
+  hipStream_t st = c->st;
+  m.n = n;
+  m.P = P;
+  m.rank.assign((size_t)P * P, 0);
+  m.where.assign((size_t)P * P, nullptr);
+  // centres through the reader's Cartesian -> spherical -> Cartesian round trip
+  std::vector<double> ctr(3 * (size_t)P);
+  for(int i = 0; i < P; ++i) {
+    const double x = c->xyz[3 * i], y = c->xyz[3 * i + 1], z = c->xyz[3 * i + 2];
+    const double r = std::sqrt(x * x + y * y + z * z);
+    const double th = r > 0.0 ? std::acos(z / r) : 0.0, ph = r > 0.0 ? std::atan2(y, x) : 0.0;
+    ctr[3 * i] = r * std::sin(th) * std::cos(ph);
+    ctr[3 * i + 1] = r * std::sin(th) * std::sin(ph);
+    ctr[3 * i + 2] = r * std::cos(th);
+  }
+  std::vector<int> dl, al;
+  for(int i = 0; i < P; ++i)
+    for(int j = 0; j < P; ++j) {
+      if(i == j) continue;
+      const double d = std::sqrt(std::pow(ctr[3 * j] - ctr[3 * i], 2.0) + std::pow(ctr[3 * j + 1] - ctr[3 * i + 1], 2.0) +
+                                 std::pow(ctr[3 * j + 2] - ctr[3 * i + 2], 2.0));
+      if(d >= 2.0 * (c->radius[i] + c->radius[j])) {
+        al.push_back(i * P + j);
+      } else {
+        dl.push_back(i * P + j);
+        m.rank[(size_t)i * P + j] = -1;
+      }
+    }
+  m.ndense = (int)dl.size();
+  m.naca = (int)al.size();
+  cplx k = c->waveK();
+  if(shf) k = omx::mk(2.0 * k.x, 2.0 * k.y);  // PreconditionedMatrix.cpp:725
+  omx::PairArgs base{P, nmax, shf ? omx::OV_TSH : omx::OV_T, k, c->d_sph.as<double>(), c->d_ov.as<cplx>(),
+                     c->d_coef.as<double>(), (size_t)n, nullptr, nullptr};
+  base.zblk = n2;
+  const size_t listn = std::max(dl.size(), (size_t)1);
+  if(m.ndense) {
+    m.dense.ensure(sizeof(cplx) * n2 * dl.size(), T);
+    c->d_plist.ensure(sizeof(int) * listn, T);
+    HIPCHK(hipMemcpyAsync(c->d_plist.p, dl.data(), sizeof(int) * dl.size(), hipMemcpyHostToDevice, st));
+    omx::PairArgs pa = base;
+    pa.S64 = m.dense.as<cplx>();
+    pa.plist = c->d_plist.as<int>();
+    pa.nlist = m.ndense;
+    omx::run_pairs(pa, st);
+    for(size_t b = 0; b < dl.size(); ++b) m.where[dl[b]] = m.dense.as<cplx>() + b * n2;
+    HIPCHK(hipStreamSynchronize(st));  // dl and the list buffer are reused below
+  }
+  if(m.naca) {
+    const int C = (int)std::max<double>(1.0, std::min<double>(m.naca, kAcaStagingBytes / (48.0 * n2)));
+    c->d_acaA.ensure(sizeof(cplx) * n2 * C, T);
+    c->d_acaU.ensure(sizeof(cplx) * n2 * C, T);
+    c->d_acaV.ensure(sizeof(cplx) * n2 * C, T);
+    c->d_acaR.ensure(sizeof(int) * C, T);
+    c->d_acaOff.ensure(sizeof(size_t) * C, T);
+    c->d_plist.ensure(sizeof(int) * std::max((size_t)C, listn), T);
+    std::vector<int> rk(C);
+    std::vector<size_t> off(C);
+    for(int b0 = 0; b0 < m.naca; b0 += C) {
+      const int nb = std::min(C, m.naca - b0);
+      HIPCHK(hipMemcpyAsync(c->d_plist.p, al.data() + b0, sizeof(int) * nb, hipMemcpyHostToDevice, st));
+      omx::PairArgs pa = base;
+      pa.S64 = c->d_acaA.as<cplx>();
+      pa.plist = c->d_plist.as<int>();
+      pa.nlist = nb;
+      omx::run_pairs(pa, st);
+      omx::AcaArgs aa{n, nb, kAcaEps, c->d_acaA.as<cplx>(), c->d_acaU.as<cplx>(), c->d_acaV.as<cplx>(),
+                      c->d_acaR.as<int>()};
+      omx::run_aca(aa, st);
+      HIPCHK(hipMemcpyAsync(rk.data(), c->d_acaR.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      size_t tot = 0;
+      for(int b = 0; b < nb; ++b) {
+        if(rk[b] < 1 || rk[b] > n) throw OmxError(OMX_ERR_DEVICE, "ACA returned rank " + std::to_string(rk[b]));
+        off[b] = tot;
+        tot += 2 * (size_t)n * rk[b];
+      }
+      m.pools.emplace_back();
+      DBuf &pool = m.pools.back();
+      pool.ensure(sizeof(cplx) * tot, T);
+      HIPCHK(hipMemcpyAsync(c->d_acaOff.p, off.data(), sizeof(size_t) * nb, hipMemcpyHostToDevice, st));
+      omx::run_aca_pack(c->d_acaU.as<cplx>(), c->d_acaV.as<cplx>(), n, nb, c->d_acaR.as<int>(), pool.as<cplx>(),
+                        c->d_acaOff.as<size_t>(), st);
+      for(int b = 0; b < nb; ++b) {
+        m.rank[al[b0 + b]] = rk[b];
+        m.where[al[b0 + b]] = pool.as<cplx>() + off[b];
+      }
+      HIPCHK(hipStreamSynchronize(st));  // off, rk and the list buffer are reused
+    }
+    // the staging is only needed while compressing
+    DBuf *stg[] = {&c->d_acaA, &c->d_acaU, &c->d_acaV};
+    for(DBuf *b : stg) b->release(T);
+  }
+  // descriptors, block row by block row
+  std::vector<omx::AcaDesc> desc;
+  std::vector<int> rowptr(P + 1, 0), acalist;
+  int toff = 0;
+  m.bytes = 0;
+  m.ref_mb = (double)P * n2 * 16.0 / 1e6;  // identity blocks
+  for(int i = 0; i < P; ++i) {
+    rowptr[i] = (int)desc.size();
+    for(int j = 0; j < P; ++j) {
+      const int r = m.rank[(size_t)i * P + j];
+      if(i == j) continue;
+      omx::AcaDesc d{m.where[(size_t)i * P + j], j, r, 0};
+      if(r > 0) {
+        d.toff = toff;
+        toff += r;
+        acalist.push_back((int)desc.size());
+        m.bytes += 32.0 * n * r;
+        m.ref_mb += 2.0 * n * r * 16.0 / 1e6;
+        m.rmax = std::max(m.rmax, r);
+        m.rsum += r;
+      } else {
+        m.bytes += 16.0 * n2;
+        m.ref_mb += n2 * 16.0 / 1e6;
+      }
+      desc.push_back(d);
+    }
+  }
+  rowptr[P] = (int)desc.size();
+  m.G = P > 1 ? std::max(1, std::min(P - 1, (2048 + P - 1) / P)) : 0;
+  m.desc.ensure(sizeof(omx::AcaDesc) * std::max<size_t>(desc.size(), 1), T);
+  m.rowptr.ensure(sizeof(int) * (P + 1), T);
+  m.acalist.ensure(sizeof(int) * std::max<size_t>(acalist.size(), 1), T);
+  m.t.ensure(sizeof(cplx) * std::max(toff, 1), T);
+  m.part.ensure(sizeof(cplx) * (size_t)std::max(m.G, 1) * P * n, T);
+  if(!desc.empty())
+    HIPCHK(hipMemcpyAsync(m.desc.p, desc.data(), sizeof(omx::AcaDesc) * desc.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(m.rowptr.p, rowptr.data(), sizeof(int) * (P + 1), hipMemcpyHostToDevice, st));
+  if(!acalist.empty())
+    HIPCHK(hipMemcpyAsync(m.acalist.p, acalist.data(), sizeof(int) * acalist.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // the host vectors end here
+  m.built = true;
+}
+
+// y = S x (V x) with the compressed operator, on stream st.
+void aca_apply(omx_context *c, bool shf, const cplx *x, cplx *y, hipStream_t st) {
+  omx_context::AcaMat &m = c->aca[shf ? 1 : 0];
+  if(!m.built) throw OmxError(OMX_ERR_STATE, "no compressed operator");
+  omx::AcaOp op{c->P, m.n, c->N, c->Np, m.G, m.desc.as<omx::AcaDesc>(), m.rowptr.as<int>(), m.acalist.as<int>(),
+                m.naca, m.t.as<cplx>(), m.part.as<cplx>()};
+  omx::aca_matvec(op, x, y, st);
 }
 
 // ------------------------------------------------------------------ GMRES
@@ -291,6 +482,10 @@ struct Gmres {
   std::function<void(const cplx *, cplx *)> prec;
   int N;
   hipStream_t s;  // the stream every step runs on
+  // an operator given as a product (set: used instead of A), and its bytes
+  // read per product
+  std::function<void(const cplx *, cplx *)> op = nullptr;
+  double op_bytes = 0.0;
   double nrm(const cplx *x) {
     omx::nrm2sq(x, N, c->d_nrm.as<double>() + 1, c->d_nrm.as<double>(), s);
     double v;
@@ -300,9 +495,14 @@ struct Gmres {
   }
   void matvec(const cplx *x, cplx *y) {
     if(c->profile) HIPCHK(hipEventRecord(c->ev[6], s));
-    omx::gemv64(A, (size_t)c->Np, N, x, y, c->d_gpart.as<cplx>(), s);
+    if(op) {
+      op(x, y);
+      c->mv_bytes += op_bytes;
+    } else {
+      omx::gemv64(A, (size_t)c->Np, N, x, y, c->d_gpart.as<cplx>(), s);
+      c->mv_bytes += 16.0 * c->Np * (double)c->Np;
+    }
     ++c->mv_calls;
-    c->mv_bytes += 16.0 * c->Np * (double)c->Np;
     if(c->profile) {
       HIPCHK(hipEventRecord(c->ev[7], s));
       HIPCHK(hipEventSynchronize(c->ev[7]));
@@ -514,7 +714,11 @@ void solve_system(omx_context *c, bool shf, const cplx *rhs, cplx *x, int *iters
   hipStream_t st = sys_stream(c, shf);
   DBuf &status = shf ? c->d_status2 : c->d_status;
   Gmres gm{c, A64, nullptr, Np, st};
-  if(c->kind == OMX_SOLVER_GMRES64) {
+  if(c->kind == OMX_SOLVER_ACA_GMRES) {  // Gmres_Zcomp on the compressed operator
+    gm.op = [c, shf, st](const cplx *in, cplx *out) { aca_apply(c, shf, in, out, st); };
+    gm.op_bytes = c->aca[shf ? 1 : 0].bytes;
+  }
+  if(!is_lu(c->kind)) {
     HIPCHK(hipMemsetAsync(status.p, 0, 16, st));
     HIPCHK(hipMemsetAsync(x, 0, sizeof(cplx) * Np, st));
     *iters = gm.solve(rhs, x, c->tol, c->maxit, c->restart, res);
@@ -1180,6 +1384,10 @@ void omx_destroy(omx_context *c) {
                    &w.flags2, &w.dinv, &w.dinv2, &w.map, &w.status, &w.basis, &w.basisZ, &w.r, &w.w1, &w.w2, &w.gpart,
                    &w.part, &w.hd, &w.nrm, &w.sc, &w.act, &w.abscoef, &w.abssh, &w.kz, &w.omz};
   for(DBuf *b : sbufs) b->release(&c->total);
+  c->aca[0].release(&c->total);
+  c->aca[1].release(&c->total);
+  DBuf *abufs[] = {&c->d_acaA, &c->d_acaU, &c->d_acaV, &c->d_acaR, &c->d_acaOff, &c->d_plist};
+  for(DBuf *b : abufs) b->release(&c->total);
   for(auto &e : c->ev)
     if(e) (void)hipEventDestroy(e);
   for(auto e : c->pev)
@@ -1263,7 +1471,7 @@ int omx_set_excitation(omx_context *c, double wavelength, double theta, double p
 
 int omx_set_solver(omx_context *c, int kind, double tol, int maxit, int restart) {
   return guard([&] {
-    if(!c || kind < 0 || kind > 2 || !(tol > 0.0) || maxit < 1 || restart < 1 || restart > 200)
+    if(!c || kind < 0 || kind > 3 || !(tol > 0.0) || maxit < 1 || restart < 1 || restart > 256)
       throw OmxError(OMX_ERR_ARG, "invalid solver settings");
     c->kind = kind;
     c->tol = tol;
@@ -1350,12 +1558,19 @@ int omx_update(omx_context *c) {
     omx::run_origin(og, c->st);
     HIPCHK(hipEventRecord(c->ev[1], c->st));
     const size_t lda = (size_t)c->Np;
+    if(c->kind == OMX_SOLVER_ACA_GMRES) {
+      aca_build(c, false);
+      if(c->sh) aca_build(c, true); else c->aca[1].release(&c->total);
+    } else {
+      c->aca[0].release(&c->total);
+      c->aca[1].release(&c->total);
+    }
     omx::PairArgs pa{P, nmax, omx::OV_T, c->waveK(), c->d_sph.as<double>(), c->d_ov.as<cplx>(),
                      c->d_coef.as<double>(), lda, c->d_S64.as<cplx>(), c->d_S32.as<float2>()};
-    omx::run_pairs(pa, c->st);
+    if(c->kind != OMX_SOLVER_ACA_GMRES) omx::run_pairs(pa, c->st);
     if(c->d_S64.p) omx::run_pad64(c->d_S64.as<cplx>(), c->N, c->Np, lda, c->st);
     if(c->d_S32.p) omx::run_pad32(c->d_S32.as<float2>(), c->N, c->Np, lda, c->st);
-    if(c->sh) {
+    if(c->sh && c->kind != OMX_SOLVER_ACA_GMRES) {
       omx::PairArgs pv = pa;
       pv.tkind = omx::OV_TSH;
       cplx k = c->waveK();
@@ -1392,7 +1607,7 @@ int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, do
     // side does: its factorisation runs on the second stream pair beside the
     // fundamental solve (not while profiling, which times each factorisation
     // alone)
-    const bool lu_kind = c->kind != OMX_SOLVER_GMRES64;
+    const bool lu_kind = is_lu(c->kind);
     if(c->sh && lu_kind && !c->factored_sh && !c->profile) {
       // V's factorisation starts once S's has passed a fraction of its
       // columns (by then S's trailing updates no longer fill the device),
@@ -1614,6 +1829,44 @@ int omx_read_matrix_block(omx_context *c, int which, int row0, int col0, int nro
   });
 }
 
+int omx_aca_info(omx_context *c, int which, int *ranks, double *info) {
+  return guard([&] {
+    if(!c || which < 0 || which > 1) throw OmxError(OMX_ERR_ARG, "invalid arguments");
+    const omx_context::AcaMat &m = c->aca[which];
+    if(!m.built) throw OmxError(OMX_ERR_STATE, "no compressed operator (OMX_SOLVER_ACA_GMRES and omx_update first)");
+    if(ranks) std::copy(m.rank.begin(), m.rank.end(), ranks);
+    if(info) {
+      info[0] = m.bytes;
+      info[1] = 16.0 * (double)c->N * c->N;
+      info[2] = m.ref_mb;
+      info[3] = m.naca;
+      info[4] = m.ndense;
+      info[5] = m.rmax;
+      info[6] = m.naca ? m.rsum / m.naca : 0.0;
+    }
+  });
+}
+
+int omx_aca_block(omx_context *c, int which, int i, int j, int *rank, double *U, double *V) {
+  return guard([&] {
+    if(!c || which < 0 || which > 1 || !rank || !U || !V) throw OmxError(OMX_ERR_ARG, "invalid arguments");
+    const omx_context::AcaMat &m = c->aca[which];
+    if(!m.built) throw OmxError(OMX_ERR_STATE, "no compressed operator (OMX_SOLVER_ACA_GMRES and omx_update first)");
+    if(i < 0 || j < 0 || i >= m.P || j >= m.P) throw OmxError(OMX_ERR_ARG, "block outside the cluster");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t b = (size_t)i * m.P + j, n = m.n;
+    const int r = m.rank[b];
+    *rank = r;
+    if(r < 0) {
+      HIPCHK(hipMemcpyAsync(U, m.where[b], sizeof(cplx) * n * n, hipMemcpyDeviceToHost, c->st));
+    } else if(r > 0) {
+      HIPCHK(hipMemcpyAsync(U, m.where[b], sizeof(cplx) * n * r, hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipMemcpyAsync(V, m.where[b] + n * r, sizeof(cplx) * n * r, hipMemcpyDeviceToHost, c->st));
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
 int omx_read_vector(omx_context *c, int which, double *out) {
   return guard([&] {
     if(!c || !c->updated || !out) throw OmxError(OMX_ERR_STATE, "nothing assembled");
@@ -1694,7 +1947,7 @@ int omx_solve_sweep(omx_context *c, int nl, const double *lambdas, const double 
   return guard([&] {
     if(!c || !c->have_geom || !c->have_exc || nl < 1 || !lambdas || batch < 0)
       throw OmxError(OMX_ERR_ARG, "invalid sweep (geometry, excitation and wavelengths required)");
-    if(c->kind == OMX_SOLVER_GMRES64) throw OmxError(OMX_ERR_ARG, "sweeps need an LU solver kind");
+    if(!is_lu(c->kind)) throw OmxError(OMX_ERR_ARG, "sweeps need an LU solver kind");
     for(int l = 0; l < nl; ++l)
       if(!(lambdas[l] > 0.0)) throw OmxError(OMX_ERR_ARG, "non-positive wavelength");
     HIPCHK(hipSetDevice(c->device));
@@ -1836,6 +2089,8 @@ int omx_trim(omx_context *c) {
                    &w.V32, &w.F64, &w.F64sh, &w.basis, &w.basisZ, &w.gpart, &c->d_basis, &c->d_basisZ,
                    &c->d_gpart};
     for(DBuf *b : big) b->release(&c->total);
+    c->aca[0].release(&c->total);
+    c->aca[1].release(&c->total);
     c->updated = c->solved = false;
     c->factored = c->factored_sh = false;
   });

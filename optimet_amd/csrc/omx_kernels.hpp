@@ -56,6 +56,11 @@ struct PairArgs {
   // batches (blockIdx.z = system z): wavenumber kz[z], ov + z zov, matrices + z zS
   const cplx *kz = nullptr;
   size_t zov = 0, zS = 0;
+  // pair list (plist non-null): nlist blocks, block b = pair plist[b] (i P + j)
+  // written to S64 / S32 + b zblk with leading dimension lda
+  const int *plist = nullptr;
+  int nlist = 0;
+  size_t zblk = 0;
 };
 
 struct OriginArgs {
@@ -108,6 +113,37 @@ void run_cross(const CrossArgs &a, hipStream_t st);
 void run_pad64(cplx *A, int N, int Np, size_t lda, hipStream_t st, int nb = 1, size_t zs = 0);
 void run_pad32(float2 *A, int N, int Np, size_t lda, hipStream_t st, int nb = 1, size_t zs = 0);
 void fill_coef(double *host);
+
+// ---------------------------------------------------------------- ACA
+struct AcaArgs {
+  int n, nblk;       // block order (<= 256) and blocks in the launch
+  double eps;        // compression tolerance
+  const cplx *A;     // nblk blocks, n x n column-major, n^2 apart
+  cplx *U, *V;       // nblk slots of n^2: U column p at U + p n, V row p at V + p n
+  int *rank;         // out: nblk
+};
+void run_aca(const AcaArgs &a, hipStream_t st);
+// block b's factors, U (n x r) then V (r x n), copied to pool + off[b]
+void run_aca_pack(const cplx *U, const cplx *V, int n, int nblk, const int *rank, cplx *pool, const size_t *off,
+                  hipStream_t st);
+
+// One off-diagonal block of a compressed operator: rank -1 dense (n x n
+// column-major at M), r > 0 compressed (U n x r at M, V r x n at M + n r,
+// V x_j kept at t + toff).
+struct AcaDesc {
+  const cplx *M;
+  int j, rank, toff;
+};
+struct AcaOp {
+  int P, n, N, Np, G;          // G groups of a block row's blocks (0 when P = 1)
+  const AcaDesc *desc;         // sorted by block row, rows at rowptr[i] .. rowptr[i+1]
+  const int *rowptr, *acalist; // acalist: desc indices of the naca compressed blocks
+  int naca;
+  cplx *t, *part;              // V x products; partial rows [G][P][n]
+};
+// y = S x for the compressed operator S (identity diagonal blocks, identity
+// on the padding rows N .. Np-1)
+void aca_matvec(const AcaOp &op, const cplx *x, cplx *y, hipStream_t st);
 
 // ---------------------------------------------------------------- LU
 constexpr int kNB = 64;        // LU block size (panel width, padding granule)

@@ -15,11 +15,14 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboptimet_mi355x.so")
 
-LU64, LU32_IR, GMRES64 = 0, 1, 2
+LU64, LU32_IR, GMRES64, ACA_GMRES = 0, 1, 2, 3
 PIVOT_BLOCK, PIVOT_PARTIAL = 0, 1
 PIVOTS = {"block": PIVOT_BLOCK, "partial": PIVOT_PARTIAL}
 GEMMS = {"bf16x3": 0, "fp32": 1}
-KINDS = {"lu64": LU64, "lu32ir": LU32_IR, "gmres64": GMRES64}
+KINDS = {"lu64": LU64, "lu32ir": LU32_IR, "gmres64": GMRES64, "aca": ACA_GMRES}
+# Gmres_Zcomp settings of the reference's ACA path: tol 1e-6, 240 steps per
+# cycle, 2 cycles (PreconditionedMatrixSolver.h:47-56)
+ACA_DEFAULTS = {"tol": 1e-6, "maxit": 480, "restart": 240}
 
 ERRORS = {-1: "argument", -2: "device", -3: "memory", -4: "singular", -5: "state", -6: "no convergence",
           -7: "overlap"}
@@ -29,7 +32,7 @@ EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "o
            "omx_cross_sections", "omx_scattering_size", "omx_get_stats", "omx_read_matrix", "omx_read_vector",
            "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_set_lu32_gemm", "omx_solve_sweep", "omx_mem_info", "omx_trim",
            "omx_dense_solve_noalloc", "omx_read_matrix_block", "omx_absorption_cross_section_sh",
-           "omx_fields"]
+           "omx_fields", "omx_aca_info", "omx_aca_block"]
 
 
 class OmxError(RuntimeError):
@@ -93,6 +96,8 @@ def load_library(path=LIB_PATH):
     L.omx_read_matrix_block.argtypes = [P, I, I, I, I, I, D]
     L.omx_absorption_cross_section_sh.argtypes = [P, D]
     L.omx_fields.argtypes = [P, I, D, D, D, D, D]
+    L.omx_aca_info.argtypes = [P, I, ctypes.POINTER(ctypes.c_int), D]
+    L.omx_aca_block.argtypes = [P, I, I, I, ctypes.POINTER(ctypes.c_int), D, D]
     _lib = L
     return L
 
@@ -185,15 +190,22 @@ class Context:
 class Solver:
     """MI355X backend behind the AbstractSolver interface (Solver.h:62-144).
 
-    kind: "lu64" (FP64 LU), "lu32ir" (complex-FP32 MFMA LU + FP64 GMRES-IR) or
-    "gmres64" (GMRES on the FP64 matrix).  pivoting: "block" (pivots within
-    each 64 x 64 diagonal block, FP64 refinement, automatic fallback) or
-    "partial" (whole-column partial pivoting; with "lu64" the plain FP64 LU).
+    kind: "lu64" (FP64 LU), "lu32ir" (complex-FP32 MFMA LU + FP64 GMRES-IR),
+    "gmres64" (GMRES on the FP64 matrix) or "aca" (<ACA compression="yes"/>:
+    the admissible blocks ACA-compressed, GMRES on the compressed operator;
+    tol, maxit and restart default to the reference's 1e-6, 480 and 240).
+    pivoting: "block" (pivots within each 64 x 64 diagonal block, FP64
+    refinement, automatic fallback) or "partial" (whole-column partial
+    pivoting; with "lu64" the plain FP64 LU).
     """
 
-    def __init__(self, cluster, kind="lu64", tol=1e-16, maxit=300, restart=40, context=None, pivoting="block"):
+    def __init__(self, cluster, kind="lu64", tol=None, maxit=None, restart=None, context=None, pivoting="block"):
         self.ctx = context or Context()
         self.kind = kind
+        d = ACA_DEFAULTS if kind == "aca" else {"tol": 1e-16, "maxit": 300, "restart": 40}
+        tol = d["tol"] if tol is None else tol
+        maxit = d["maxit"] if maxit is None else maxit
+        restart = d["restart"] if restart is None else restart
         _check(self.ctx.L.omx_set_solver(self.ctx.h, KINDS[kind], tol, maxit, restart))
         self.ctx.set_pivoting(pivoting)
         self.set_cluster(cluster)
@@ -294,6 +306,33 @@ class Solver:
         out = np.zeros(N * N, np.complex128)
         _check(self.ctx.L.omx_read_matrix(self.ctx.h, 1 if sh else 0, _d(out.view(np.float64))))
         return out.reshape(N, N).T
+
+    def aca_info(self, sh=False):
+        """The compressed operator of kind "aca" after update(): (P, P) ranks
+        (r, -1 dense, 0 diagonal) and a dict of its sizes."""
+        P = self.cluster.nobj
+        ranks = np.zeros(P * P, np.int32)
+        info = np.zeros(7)
+        _check(self.ctx.L.omx_aca_info(self.ctx.h, 1 if sh else 0, ranks.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       _d(info)))
+        keys = ("bytes", "dense_bytes", "reference_mb", "compressed_blocks", "dense_blocks", "max_rank", "mean_rank")
+        return ranks.reshape(P, P), dict(zip(keys, info.tolist()))
+
+    def aca_block(self, i, j, sh=False):
+        """Block (i, j) of the compressed operator: (U (n, r), V (r, n)) of a
+        compressed block, (block, None) of a dense one, (None, None) on the
+        diagonal."""
+        n = 2 * self.cluster.nf
+        U, V = np.zeros(n * n, np.complex128), np.zeros(n * n, np.complex128)
+        r = ctypes.c_int(0)
+        _check(self.ctx.L.omx_aca_block(self.ctx.h, 1 if sh else 0, int(i), int(j), ctypes.byref(r),
+                                        _d(U.view(np.float64)), _d(V.view(np.float64))))
+        k = r.value
+        if k < 0:
+            return U.reshape(n, n).T.copy(), None
+        if k == 0:
+            return None, None
+        return U[:n * k].reshape(k, n).T.copy(), V[:n * k].reshape(k, n).copy()
 
     def matrix_block(self, i, j, sh=False):
         """Block (row object i, column object j) of S (or V): 2nf x 2nf."""

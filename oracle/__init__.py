@@ -86,7 +86,7 @@ def _declare(L):
     L.ora_absorption_sh.argtypes = [_P, _D, _D, _D, _D]
     L.ora_fields.argtypes = [_P, _D, _D, _D, _D, _D, _I, _D, _D, _D, _D, _D]
     L.ora_time_sample.argtypes = [_P, _I, _I, _I, _D]
-    L.ora_aca_block.argtypes = [_I, _D, _Dbl, ctypes.POINTER(ctypes.c_int), _D, _D]
+    L.ora_aca_block.argtypes = [_I, _D, _Dbl, ctypes.POINTER(ctypes.c_int), _D, _D, _D]
     L.ora_aca_operator.argtypes = [_P, _I, _Dbl, _D, ctypes.POINTER(ctypes.c_int)]
     L.ora_gmres_zcomp.argtypes = [_I, _D, _D, _Dbl, _I, _I, _D, _D]
 
@@ -257,16 +257,21 @@ def dense_solve(S, b, method=0):
     return x
 
 
-def aca_block(A, eps=1e-3):
-    """ACA_compression of the square block A: (U (n, r), V (r, n))."""
+def aca_block(A, eps=1e-3, trace=False):
+    """ACA_compression of the square block A: (U (n, r), V (r, n)) and, with
+    trace, the (r, 3) per-step ratios: runner-up / largest modulus of the
+    pivot column's choice and of the next row's choice, and the stopping
+    ratio (>= 1 stops)."""
     n = A.shape[0]
     flat = np.asfortranarray(A, dtype=np.complex128).reshape(-1, order="F").copy()
     U, V = _cz(n * n), _cz(n * n)
+    tr = np.zeros(3 * n)
     r = ctypes.c_int(0)
     _check(lib().ora_aca_block(n, _ptr(flat.view(np.float64)), eps, ctypes.byref(r), _ptr(U.view(np.float64)),
-                               _ptr(V.view(np.float64))))
+                               _ptr(V.view(np.float64)), _ptr(tr)))
     k = r.value
-    return U[:n * k].reshape(k, n).T.copy(), V[:n * k].reshape(k, n).copy()
+    out = (U[:n * k].reshape(k, n).T.copy(), V[:n * k].reshape(k, n).copy())
+    return out + (tr[:3 * k].reshape(k, 3),) if trace else out
 
 
 def aca_operator(prob, sh=False, eps=1e-3):

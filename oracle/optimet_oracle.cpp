@@ -776,8 +776,8 @@ void dense_solve(int N, const std::vector<cd> &S, const std::vector<cd> &b, std:
 // modulus among the entries whose index is not one of used[0 .. nused-1];
 // -1 when no such entry is non-zero (the reference then returns an unset
 // index).
-int max_index(const cd *v, int n, const std::vector<int> &used, int nused) {
-  double mx = 0.0;
+int max_index(const cd *v, int n, const std::vector<int> &used, int nused, double *tie = nullptr) {
+  double mx = 0.0, second = 0.0;
   int im = -1;
   for(int i = 0; i < n; ++i) {
     bool same = false;
@@ -786,10 +786,14 @@ int max_index(const cd *v, int n, const std::vector<int> &used, int nused) {
     if(same) continue;
     double a = std::abs(v[i]);
     if(a > mx) {
+      second = mx;
       mx = a;
       im = i;
+    } else if(a > second) {
+      second = a;
     }
   }
+  if(tie) *tie = mx > 0.0 ? second / mx : 0.0;  // how close the runner-up came
   return im;
 }
 
@@ -805,8 +809,10 @@ int max_index(const cd *v, int n, const std::vector<int> &used, int nused) {
 // the largest entry of the residual column among the unused rows.  When a
 // residual row or column has no non-zero candidate left the k terms
 // reproduce A exactly and the iteration stops (the reference reads an unset
-// index there).  Returns r.
-int aca_block(int n, const cd *A, cd *U, cd *V, double eps) {
+// index there).  Returns r.  trace (optional, 3 per step): the runner-up /
+// largest modulus of the pivot column's choice, of the next row's choice,
+// and eps sqrt(estimate) / (|u_k| |v_k|) (the test stops at >= 1).
+int aca_block(int n, const cd *A, cd *U, cd *V, double eps, double *trace = nullptr) {
   std::vector<int> I(1, 0), J;
   std::vector<cd> row(n);
   auto nrm = [n](const cd *v) {
@@ -823,7 +829,9 @@ int aca_block(int n, const cd *A, cd *U, cd *V, double eps) {
       for(int p = 0; p < k; ++p) s += U[(size_t)p * n + ik] * V[(size_t)p * n + c];
       row[c] = A[(size_t)c * n + ik] - s;
     }
-    const int jk = max_index(row.data(), n, J, k);
+    double tie = 0.0;
+    const int jk = max_index(row.data(), n, J, k, &tie);
+    if(trace) trace[3 * k] = tie, trace[3 * k + 1] = 0.0, trace[3 * k + 2] = 0.0;
     if(jk < 0) break;
     J.push_back(jk);
     const cd piv = row[jk];
@@ -849,9 +857,10 @@ int aca_block(int n, const cd *A, cd *U, cd *V, double eps) {
         sum += std::abs(p1) * std::abs(p2);
       }
       norma = norma + cn * cn * (rn * rn) + 2.0 * sum;
+      if(trace) trace[3 * k + 2] = eps * std::sqrt(norma) / (cn * rn);
       if(eps * std::sqrt(norma) >= cn * rn) break;
     }
-    const int in = max_index(uk, n, I, k + 1);
+    const int in = max_index(uk, n, I, k + 1, trace ? &trace[3 * k + 1] : nullptr);
     if(in < 0) break;
     I.push_back(in);
   }
@@ -1674,10 +1683,11 @@ int ora_dense_solve(int N, const double *S, const double *b, double *x, int meth
   });
 }
 
-int ora_aca_block(int n, const double *A, double eps, int *rank, double *U, double *V) {
+int ora_aca_block(int n, const double *A, double eps, int *rank, double *U, double *V, double *trace) {
   return guard([&] {
     if(n < 1) throw OraError("block order must be positive");
-    *rank = aca_block(n, reinterpret_cast<const cd *>(A), reinterpret_cast<cd *>(U), reinterpret_cast<cd *>(V), eps);
+    *rank = aca_block(n, reinterpret_cast<const cd *>(A), reinterpret_cast<cd *>(U), reinterpret_cast<cd *>(V), eps,
+                      trace);
   });
 }
 

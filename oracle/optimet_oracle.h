@@ -130,8 +130,10 @@ int ora_fields(const ora_problem *pb, const double *tables, const double *x_sca,
 
 /* ACA_compression of the n x n block A (column-major) at tolerance eps:
  * A ~ U V with U n x r (column-major, n x n capacity) and V r x n (row-major,
- * n x n capacity); *rank = r. */
-int ora_aca_block(int n, const double *A, double eps, int *rank, double *U, double *V);
+ * n x n capacity); *rank = r.  trace (3 n doubles, may be NULL): per step the
+ * runner-up / largest candidate modulus of the pivot column's choice and of
+ * the next row's choice, and the stopping ratio eps sqrt(estimate) / (|u| |v|). */
+int ora_aca_block(int n, const double *A, double eps, int *rank, double *U, double *V, double *trace);
 
 /* Scattering_matrix_ACA_FF (sh=0) / _SH (sh=1) expanded to the N x N
  * compressed operator S (column-major): admissible blocks (i != j,
