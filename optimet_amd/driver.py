@@ -1,8 +1,11 @@
 """Case driver: OPTIMET's XML input and its outputs, over the MI355X backend.
 
-``python -m optimet_amd.driver case.xml [--kind lu32ir|lu64|gmres64]`` does
-what ``Optimet3D case`` does for a sphere cluster (main.cpp ->
-Simulation::run, Simulation.cpp:33-63):
+``python -m optimet_amd.driver case.xml [--kind lu32ir|lu64|gmres64|aca]``
+does what ``Optimet3D case`` does for a sphere cluster (main.cpp ->
+Simulation::run, Simulation.cpp:33-63).  ``<ACA compression="yes"/>`` selects
+the ACA-compressed operator with the reference's GMRES settings
+(PreconditionedMatrixSolver.h:47-56), as the reference's solve() does; without
+it the mixed-precision LU (or the --kind given) solves the dense systems.
 
 * ``<output type="response">`` with ``<scan><wavelength initial= final=
   stepsize=/></scan>``: the wavelength scan of Simulation::scan_wavelengths
@@ -277,6 +280,10 @@ def read_case(path):
         raise CaseError("Output not defined!")
     otype = out.get("type")
     if otype == "response":
+        if out.find("scan/radius") is not None:
+            # outputType 12 / 112 (Reader.cpp:893-902): Simulation::run has no
+            # branch for them and prints "Nothing to do?" (Simulation.cpp:59-62)
+            raise CaseError("radius scans are not run by the reference (Simulation::run has no branch for them)")
         wl = out.find("scan/wavelength")
         if wl is None:
             raise CaseError("a response output needs <scan><wavelength .../></scan>")
@@ -378,7 +385,8 @@ def scan(case, solver_, batch=0):
     when the susceptibilities vary too (the gold model)."""
     lams = case.wavelengths
     per = [case.at(l) for l in lams]
-    if not any(s.material.varying_susceptibilities for s in case.spheres):
+    batched = solver_.kind != "aca" and not any(s.material.varying_susceptibilities for s in case.spheres)
+    if batched:
         solver_.set_cluster(per[0])
         eps = np.array([c.eps_r for c in per])
         eps_sh = np.array([c.eps_r_sh for c in per])
@@ -394,12 +402,14 @@ def scan(case, solver_, batch=0):
     return xs
 
 
-def run_case(case, prefix, kind="lu32ir", context=None, batch=0):
+def run_case(case, prefix, kind=None, context=None, batch=0):
     """Solves the case on the device and writes its outputs; returns the
     written paths and the computed data (xs for a scan, the fields for a
-    field map)."""
+    field map).  kind None: "aca" when the case asks for ACA compression,
+    else "lu32ir"."""
     from . import solver
 
+    kind = kind or ("aca" if case.aca else "lu32ir")
     ctx = context or solver.Context()
     try:
         if case.output == "response":
@@ -437,7 +447,8 @@ def run_case(case, prefix, kind="lu32ir", context=None, batch=0):
 def main(argv=None):
     ap = argparse.ArgumentParser(description="Solve an OPTIMET case file on the MI355X backend")
     ap.add_argument("case", help="the XML case file (with or without .xml)")
-    ap.add_argument("--kind", default="lu32ir", choices=("lu32ir", "lu64", "gmres64"))
+    ap.add_argument("--kind", default=None, choices=("lu32ir", "lu64", "gmres64", "aca"),
+                    help="solver (default: aca when the case asks for ACA compression, else lu32ir)")
     ap.add_argument("--out", default=None, help="output prefix (default: the case path without .xml)")
     args = ap.parse_args(argv)
     path = args.case if args.case.endswith(".xml") else args.case + ".xml"

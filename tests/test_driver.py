@@ -86,6 +86,13 @@ def test_scan_count(tmp_path, li, lf, ls, n):
     assert c.wavelengths[-1] == pytest.approx(lf * 1e-9, rel=1e-14)
 
 
+def test_radius_scan_refused(tmp_path):
+    """outputType 12 / 112 have no branch in Simulation::run (Simulation.cpp:45-63)."""
+    t = text().replace('</scan>', '<radius initial="40" final="60" steps="3" /></scan>')
+    with pytest.raises(driver.CaseError, match="radius scans"):
+        driver.read_case(write(tmp_path, t))
+
+
 def test_single_wavelength_scan_refused(tmp_path):
     with pytest.raises(driver.CaseError, match="single wavelength"):
         driver.read_case(write(tmp_path, text(li=600, lf=650, ls=100)))

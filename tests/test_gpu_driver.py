@@ -34,18 +34,25 @@ def read_dat(path):
     return np.loadtxt(path, delimiter="\t", ndmin=2)
 
 
-@pytest.mark.parametrize("name", ["three_si_scan", "gold_pair_scan", "cube_lattice_scan"])
+@pytest.mark.parametrize("name", ["three_si_scan", "gold_pair_scan", "cube_lattice_scan", "aca_cluster_scan"])
 def test_scan_case(ctx, ora, tmp_path, name):
+    """A case with <ACA compression="yes"/> runs the compressed operator with
+    the reference's GMRES settings; its cross sections are held to 2e-2 of
+    the oracle's ACA path and of the dense solution (two compressions part at
+    rounding-level pivot ties and then differ by the compression error; the
+    step-for-step parity of the compression is tests/test_gpu_aca.py's)."""
     case = driver.read_case(os.path.join(CASES, name + ".xml"))
     print(f"case {name}: {case.cluster.nobj} spheres, nmax {case.nmax}, {len(case.wavelengths)} wavelengths",
           flush=True)
     prefix = str(tmp_path / name)
-    paths, xs = driver.run_case(case, prefix, kind="lu32ir", context=ctx)
+    paths, xs = driver.run_case(case, prefix, context=ctx)
     nk = 5 if case.sh else 3
     for i, lam in enumerate(case.wavelengths):
-        ref = ora.solve_all(case.at(lam))
-        for k in range(nk):
-            assert xs[i, k] == pytest.approx(ref[KEYS[k]], rel=1e-9), (name, lam, KEYS[k])
+        refs = [ora.solve_all(case.at(lam), method=2), ora.solve_all(case.at(lam))] if case.aca \
+            else [ora.solve_all(case.at(lam))]
+        for ref in refs:
+            for k in range(nk):
+                assert xs[i, k] == pytest.approx(ref[KEYS[k]], rel=2e-2 if case.aca else 1e-9), (name, lam, KEYS[k])
     expect = {"_AbsorptionCS_FF.dat": xs[:, 0] - xs[:, 1], "_ScatteringCS_FF.dat": xs[:, 1]}
     if case.sh:
         expect.update({"_ScatteringCS_SH.dat": xs[:, 3], "_AbsorptionCS_SH.dat": xs[:, 4]})
@@ -63,7 +70,7 @@ def test_field_case(ctx, ora, tmp_path, name):
     case = driver.read_case(os.path.join(CASES, name + ".xml"))
     print(f"case {name}: grid {case.grid}", flush=True)
     prefix = str(tmp_path / name)
-    paths, f = driver.run_case(case, prefix, kind="lu32ir", context=ctx)
+    paths, f = driver.run_case(case, prefix, context=ctx)
     assert paths == [prefix + "_FF.npz", prefix + "_SH.npz"]
     c = case.cluster
     pts, shape = driver.grid_points(case.grid)
