@@ -421,8 +421,8 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this single-GPU build "
-              "(the multi-GPU layout is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build runs on one GPU "
+              "(the path would shard by block rows of S and whole systems per GPU)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "config3":
