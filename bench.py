@@ -135,14 +135,14 @@ def run_sweep(cluster, lams, kind, steps, warmup, batch):
     return {"lat": lat, "total": total, "out": out, "stats": st, "prof": prof}
 
 
-def matvec_roofline(st):
-    """The refinement's FP64 matvec (gemv_part_kernel + its reduction): its
-    algorithmic bytes (16 Np^2 per system per product) over the HIP-event time
-    of its launches on the stream they run on."""
+def matvec_roofline(st, kernel="gemv_part_kernel (batched FP64 matvec of the refinement) + gemv_reduce_kernel"):
+    """The Krylov or refinement matvecs: their algorithmic bytes (16 Np^2 per
+    FP64 product, 8 Np^2 per FP32-matrix product, per system) over the
+    HIP-event time of their launches on the stream they run on."""
     if st["t_matvec_ms"] <= 0 or st["matvec_calls"] <= 0:
         return None
     gbs = st["matvec_bytes"] / (st["t_matvec_ms"] * 1e-3) / 1e9
-    return {"bound": "hbm", "kernel": "gemv_part_kernel (batched FP64 matvec of the refinement) + gemv_reduce_kernel",
+    return {"bound": "hbm", "kernel": kernel,
             "achieved": gbs, "peak": HBM_GBS, "unit": "GB/s", "frac": gbs / HBM_GBS, "traffic": None,
             "launches": st["matvec_calls"], "avg_launch_us": 1e3 * st["t_matvec_ms"] / st["matvec_calls"],
             "bytes_per_launch": st["matvec_bytes"] / st["matvec_calls"]}
@@ -189,9 +189,12 @@ def sweep_main(args):
 
 def config4_main(args):
     """Config 4: 512 Drude spheres, nmax 6 (N = 49152, a 38.7 GB FP64 matrix),
-    fundamental frequency: unpreconditioned FP64 GMRES (HBM-bound matvec)
-    against the mixed-precision LU + GMRES-IR, and the ACA-compressed operator
-    with the reference's GMRES settings (tol 1e-6)."""
+    fundamental frequency.  Legs: the default solver choice ("auto": the mixed
+    Krylov within the mixed LU's cost, falling back to the mixed LU), FP64
+    GMRES, the mixed Krylov, the mixed LU + GMRES-IR, the FP64 LU (the
+    comparison path of the metric) and the ACA-compressed operator with the
+    reference's GMRES settings (tol 1e-6).  The Krylov paths run to the same
+    final tolerance, 1e-10."""
     from optimet_amd import solver
     from optimet_amd.problem import config4
 
@@ -203,7 +206,7 @@ def config4_main(args):
         if kind == "aca":  # <ACA compression="yes"/> with the reference's GMRES settings
             s = solver.Solver(c, kind=kind, context=ctx)
         else:
-            s = solver.Solver(c, kind=kind, context=ctx, tol=1e-10 if kind == "gmres64" else 1e-16, maxit=800)
+            s = solver.Solver(c, kind=kind, context=ctx, tol=1e-10 if kind.startswith("gmres") else 1e-16, maxit=800)
         ctx.set_profile(True)
         t0 = time.perf_counter()
         s.update()
@@ -224,44 +227,50 @@ def config4_main(args):
                      "residual": st["residual"], "matvec_calls": st["matvec_calls"],
                      "matvec_GBs": (st["matvec_bytes"] / (st["t_matvec_ms"] * 1e-3) / 1e9)
                      if st["t_matvec_ms"] > 0 else None,
+                     "fallbacks": st["pivot_fallbacks"],
                      "out": out, "cext": cs["cext"], "stats": st, "aca": aca}
-        if kind == "lu32ir":
-            roof = roofline(st, "lu32ir", SPLIT_TFLOPS)
-            res[kind]["roofline"] = roof
-    if len(legs) < 3:  # a partial run (profiling): the legs' own figures only
-        print(json.dumps({k: {f: v[f] for f in ("update_s", "solve_s", "iterations", "residual", "matvec_GBs", "aca")}
-                          for k, v in res.items()}), flush=True)
+        if kind in ("lu32ir", "lu64"):
+            res[kind]["roofline"] = roofline(st, kind, SPLIT_TFLOPS if kind == "lu32ir" else FP64_MFMA_TFLOPS)
+    keys = ("update_s", "solve_s", "converged", "iterations", "residual", "matvec_calls", "matvec_GBs", "cext")
+    if not all(k in res for k in ("auto", "lu32ir", "lu64")):  # a partial run (profiling): the legs' figures only
+        print(json.dumps({k: {f: v[f] for f in keys + ("aca",)} for k, v in res.items()}), flush=True)
         return
-    g, m, a = res["gmres64"], res["lu32ir"], res["aca"]
-    rel = (float(np.linalg.norm(m["out"]["x_sca"] - g["out"]["x_sca"]) / np.linalg.norm(g["out"]["x_sca"]))
-           if g["out"] is not None and m["out"] is not None else None)
-    ai = a["aca"]
-    aca_line = {k: a[k] for k in ("update_s", "solve_s", "converged", "iterations", "residual", "matvec_calls",
-                                  "matvec_GBs", "cext")}
-    aca_line.update({"compressed_GB": ai["bytes"] / 1e9, "dense_GB": ai["dense_bytes"] / 1e9,
-                     "fraction_of_dense": ai["bytes"] / ai["dense_bytes"], "reference_size_MB": ai["reference_mb"],
-                     "compressed_blocks": int(ai["compressed_blocks"]), "dense_blocks": int(ai["dense_blocks"]),
-                     "mean_rank": ai["mean_rank"], "max_rank": int(ai["max_rank"]),
-                     "device_GB": a["stats"]["device_bytes"] / 1e9,
-                     "rel_diff_x_sca_vs_mixed_lu": float(np.linalg.norm(a["out"]["x_sca"] - m["out"]["x_sca"])
-                                                         / np.linalg.norm(m["out"]["x_sca"]))
-                     if a["out"] is not None and m["out"] is not None else None})
+    total = {k: v["update_s"] + v["solve_s"] for k, v in res.items()}
+    ref = res["lu64"]["out"]["x_sca"]
+    rel_x = {k: float(np.linalg.norm(v["out"]["x_sca"] - ref) / np.linalg.norm(ref))
+             for k, v in res.items() if v["out"] is not None}
+    au = res["auto"]
+    picked = "gmres32ir" if au["fallbacks"] == 0 and c.N >= 16384 else "lu32ir"
     line = {
         "metric": "cluster solves/sec + p99 latency, mixed-precision vs FP64 LU at rel err <=1e-9",
-        "value": 1.0 / (m["update_s"] + m["solve_s"]), "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-        "ms_per_step": 1e3 * (m["update_s"] + m["solve_s"]), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "c64 LU (FP32-accurate bf16x3 MFMA products) + c128 GMRES-IR", "data": "synthetic (seeded Drude cluster)",
+        "value": 1.0 / total["auto"], "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1e3 * total["auto"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c128 flexible GMRES over complex-FP32 GMRES cycles (FP32 matrix, FP64 vectors)"
+        if picked == "gmres32ir" else "c64 LU (FP32-accurate bf16x3 MFMA products) + c128 GMRES-IR",
+        "data": "synthetic (seeded Drude cluster)",
         "config": {"workload": c.meta["name"], "P": c.nobj, "nmax": c.nmax, "N": c.N, "frequencies": "omega",
-                   "solver": "lu32ir"},
-        "roofline": m.get("roofline"),
-        "mixed_lu": {k: m[k] for k in ("update_s", "solve_s", "converged", "iterations", "residual", "matvec_calls",
-                                        "matvec_GBs", "cext")},
-        "gmres64": {k: g[k] for k in ("update_s", "solve_s", "converged", "iterations", "residual", "matvec_calls",
-                                       "matvec_GBs", "cext")},
-        "aca": aca_line,
-        "rel_diff_x_sca_mixed_vs_gmres": rel,
-        "device_GB": m["stats"]["device_bytes"] / 1e9,
+                   "solver": "auto", "picked": picked,
+                   "rule": "N >= 16384: the mixed Krylov within a budget of the mixed LU's cost (0.0143 N FP32 "
+                           "products), else or when it does not converge there, the mixed LU + GMRES-IR"},
+        "roofline": matvec_roofline(au["stats"], "gemv_part_kernel<float2> (inner products on the FP32 copy) and "
+                                    "gemv_part_kernel<cplx> (outer FP64 products), with gemv_reduce_kernel")
+        if picked == "gmres32ir" else res["lu32ir"].get("roofline"),
+        "speedup_vs_fp64": total["lu64"] / total["auto"],
+        "speedup_mixed_lu_vs_fp64_lu": total["lu64"] / total["lu32ir"],
+        "rel_err_x_sca_vs_fp64_lu": rel_x,
+        "legs": {k: {f: v[f] for f in keys} for k, v in res.items() if k != "aca"},
+        "lu_rooflines": {k: res[k].get("roofline") for k in ("lu32ir", "lu64")},
     }
+    if "aca" in res:
+        a = res["aca"]
+        ai = a["aca"]
+        line["aca"] = {f: a[f] for f in keys}
+        line["aca"].update({"compressed_GB": ai["bytes"] / 1e9, "dense_GB": ai["dense_bytes"] / 1e9,
+                            "fraction_of_dense": ai["bytes"] / ai["dense_bytes"],
+                            "reference_size_MB": ai["reference_mb"], "compressed_blocks": int(ai["compressed_blocks"]),
+                            "dense_blocks": int(ai["dense_blocks"]), "mean_rank": ai["mean_rank"],
+                            "max_rank": int(ai["max_rank"]), "device_GB": a["stats"]["device_bytes"] / 1e9,
+                            "rel_diff_x_sca_vs_fp64_lu": rel_x.get("aca")})
     print(json.dumps(line), flush=True)
 
 
@@ -397,7 +406,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 LU comparison path")
-    ap.add_argument("--legs", default="gmres64,lu32ir,aca",
+    ap.add_argument("--legs", default="auto,gmres64,gmres32ir,lu32ir,lu64,aca",
                     help="config4: solver paths to run (a subset prints their figures only)")
     ap.add_argument("--workload", choices=("config2", "config3", "config4", "config5"), default="config2",
                     help="config2: one 64-sphere cluster (headline); config3: 256-wavelength batched sweep; "

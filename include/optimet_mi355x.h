@@ -60,6 +60,13 @@ extern "C" {
                                   (PreconditionedMatrixSolver.h:47-56, 84-96; the
                                   reference's settings are tol 1e-6, 240 steps per cycle,
                                   2 cycles) */
+#define OMX_SOLVER_GMRES32_IR 4 /* mixed Krylov: FP64 flexible GMRES on S whose preconditioner
+                                   is one cycle of GMRES on the complex-FP32 copy of S (FP32
+                                   matrix, FP64 vectors and sums) */
+#define OMX_SOLVER_AUTO 5      /* by size: N >= 16384 starts with OMX_SOLVER_GMRES32_IR within a
+                                  budget of the mixed LU's cost and falls back to
+                                  OMX_SOLVER_LU32_IR on the same matrices; smaller systems use
+                                  OMX_SOLVER_LU32_IR (sweeps always batch the LU) */
 
 /* Pivot search of the LU factorisations (omx_set_pivoting). */
 #define OMX_PIVOT_BLOCK 0     /* among the rows of each 64 x 64 diagonal block, then
@@ -130,6 +137,12 @@ int omx_set_excitation(omx_context *ctx, double wavelength, double theta, double
  * parts (GMRES-IR refinement, GMRES), maxit their iteration cap, restart the
  * GMRES restart length (<= 256). */
 int omx_set_solver(omx_context *ctx, int kind, double tol, int maxit, int restart);
+
+/* The Krylov budget of OMX_SOLVER_AUTO in outer steps (each at most 31
+ * products); 0 (default) takes the cost model: 0.0143 N products, what the
+ * mixed LU would cost.  No reference counterpart: a tuning knob of the
+ * automatic choice. */
+int omx_set_auto_budget(omx_context *ctx, int steps);
 
 /* Pivot search of the LU solvers (default OMX_PIVOT_BLOCK).  The reference
  * solves with Eigen's ColPivHouseholderQR (PreconditionedMatrixSolver.h:58,

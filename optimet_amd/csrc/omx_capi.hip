@@ -65,6 +65,22 @@ struct DBuf {
 // residual band: 1000 units of FP64 roundoff.
 const double kBackwardErrorBound = 1e3 * 1.1e-16;
 
+// The inner solve of OMX_SOLVER_GMRES32_IR: one GMRES cycle of at most
+// kInnerSteps steps on the complex-FP32 copy, stopped at kInnerTol relative
+// residual (the FP32 matrix holds S to about 6e-8, so the inner solve can
+// gain about six digits per outer step).
+constexpr int kInnerSteps = 30;
+constexpr double kInnerTol = 1e-6;
+
+// OMX_SOLVER_AUTO: systems of order N >= kAutoKrylovN start with the mixed
+// Krylov, within a budget of outer steps whose matvecs cost what the mixed LU
+// would (8/3 N^3 flop at about 140 TFLOP/s against an 8 N^2-byte product at
+// about 6 TB/s: 0.0143 N products, kInnerSteps + 1 of them per outer step),
+// and fall back to the mixed LU + refinement on the same matrices when it
+// does not converge; smaller systems go to the LU directly.
+constexpr int kAutoKrylovN = 16384;
+inline int auto_krylov_steps(int N) { return std::max(4, (int)(0.0143 * N / (kInnerSteps + 1))); }
+
 const double kMu0 = 4.0 * omx::kPi * 1e-7;
 const double kEps0 = 1.0 / (kMu0 * omx::kC0 * omx::kC0);
 
@@ -87,6 +103,8 @@ struct omx_context {
   int sh = 0;
   // solver
   int kind = OMX_SOLVER_LU64;
+  int kind_req = OMX_SOLVER_LU64;  // as set; OMX_SOLVER_AUTO resolves into kind at each update
+  int auto_budget = 0;             // outer Krylov steps of OMX_SOLVER_AUTO (0: the cost model)
   int pivot = OMX_PIVOT_BLOCK;
   // fraction of S's columns factored before V's factorisation starts
   // (OMX_SH_OVERLAP overrides; 0 starts both together)
@@ -156,6 +174,19 @@ struct omx_context {
     }
   } aca[2];
   DBuf d_acaA, d_acaU, d_acaV, d_acaR, d_acaOff, d_plist;
+  // the inner GMRES of OMX_SOLVER_GMRES32_IR: its own basis, vectors and pinned staging
+  struct GmresWs {
+    DBuf basis, r, w1, w2, h, part, nrm, gpart;
+    cplx *h_pin = nullptr;
+    int h_pin_n = 0;
+    void release(size_t *T) {
+      DBuf *b[] = {&basis, &r, &w1, &w2, &h, &part, &nrm, &gpart};
+      for(DBuf *x : b) x->release(T);
+      if(h_pin) (void)hipHostFree(h_pin);
+      h_pin = nullptr;
+      h_pin_n = 0;
+    }
+  } ws_in;
   omx_stats stats{};
   void *scratch = nullptr;  // this device's address of the no-allocation solve's scratch array
   long long mv_calls = 0;  // FP64 matrix-vector products since the last omx_set_profile
@@ -186,7 +217,7 @@ template <typename F> int guard(F &&f) {
 
 cplx C(const double *p, int i) { return omx::mk(p[2 * i], p[2 * i + 1]); }
 
-bool use32(int kind) { return kind == OMX_SOLVER_LU32_IR; }
+bool use32(int kind) { return kind == OMX_SOLVER_LU32_IR || kind == OMX_SOLVER_GMRES32_IR; }
 bool is_lu(int kind) { return kind == OMX_SOLVER_LU64 || kind == OMX_SOLVER_LU32_IR; }
 
 void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags, DBuf &dinv, bool second) {
@@ -294,9 +325,24 @@ void alloc_matrices(omx_context *c) {
   if(f64) c->d_F64.ensure(sizeof(cplx) * n2, T); else c->d_F64.release(T);
   if(f64 && c->sh) c->d_F64sh.ensure(sizeof(cplx) * n2, T); else c->d_F64sh.release(T);
   c->d_basis.ensure(sizeof(cplx) * (size_t)c->Np * (c->restart + 1), T);
-  // preconditioned (flexible) GMRES keeps M^{-1} v_j: the LU kinds only
-  if(is_lu(c->kind)) c->d_basisZ.ensure(sizeof(cplx) * (size_t)c->Np * (c->restart + 1), T);
+  // preconditioned (flexible) GMRES keeps M^{-1} v_j: the LU kinds and the mixed Krylov
+  const bool flex = is_lu(c->kind) || c->kind == OMX_SOLVER_GMRES32_IR;
+  if(flex) c->d_basisZ.ensure(sizeof(cplx) * (size_t)c->Np * (c->restart + 1), T);
   else c->d_basisZ.release(T);
+  if(c->kind == OMX_SOLVER_GMRES32_IR) {
+    auto &w = c->ws_in;
+    const size_t Np = (size_t)c->Np;
+    w.basis.ensure(sizeof(cplx) * Np * (kInnerSteps + 1), T);
+    w.r.ensure(sizeof(cplx) * Np, T);
+    w.w1.ensure(sizeof(cplx) * Np, T);
+    w.w2.ensure(sizeof(cplx) * Np, T);
+    w.h.ensure(sizeof(cplx) * 2 * (kInnerSteps + 2), T);
+    w.part.ensure(sizeof(cplx) * 128 * (kInnerSteps + 2), T);
+    w.nrm.ensure(sizeof(double) * 256, T);
+    w.gpart.ensure(sizeof(cplx) * Np * omx::kGemvSplitMax, T);
+  } else {
+    c->ws_in.release(T);
+  }
   alloc_lu(c, c->lu, c->d_piv, c->d_perm, c->d_flags, c->d_dinv, false);
   if(c->sh) alloc_lu(c, c->lu_sh, c->d_piv2, c->d_perm2, c->d_flags2, c->d_dinv2, true);
 }
@@ -486,10 +532,16 @@ struct Gmres {
   // read per product
   std::function<void(const cplx *, cplx *)> op = nullptr;
   double op_bytes = 0.0;
+  // a workspace of its own (null: the context's); one_cycle: x starts at 0
+  // (the caller zeroes it) and a single cycle runs, with no residual product
+  // before or after it (a fixed-cost preconditioner)
+  omx_context::GmresWs *ws = nullptr;
+  bool one_cycle = false;
   double nrm(const cplx *x) {
-    omx::nrm2sq(x, N, c->d_nrm.as<double>() + 1, c->d_nrm.as<double>(), s);
+    DBuf &dn = ws ? ws->nrm : c->d_nrm;
+    omx::nrm2sq(x, N, dn.as<double>() + 1, dn.as<double>(), s);
     double v;
-    HIPCHK(hipMemcpyAsync(&v, c->d_nrm.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&v, dn.p, sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return std::sqrt(v);
   }
@@ -516,9 +568,13 @@ struct Gmres {
   int solve(const cplx *b, cplx *x, double tol, int maxit, int m, double *res) {
     // flexible form: the preconditioned vectors M^{-1} v_j are kept (Zb), so
     // the end of a cycle needs no further preconditioner application
-    cplx *V = c->d_basis.as<cplx>(), *Zb = c->d_basisZ.as<cplx>();
-    cplx *r = c->d_r.as<cplx>(), *w = c->d_w1.as<cplx>(), *z = c->d_w2.as<cplx>();
-    cplx *hd = c->d_h.as<cplx>(), *part = c->d_part.as<cplx>();
+    cplx *V = (ws ? ws->basis : c->d_basis).as<cplx>(), *Zb = c->d_basisZ.as<cplx>();
+    cplx *r = (ws ? ws->r : c->d_r).as<cplx>(), *w = (ws ? ws->w1 : c->d_w1).as<cplx>();
+    cplx *z = (ws ? ws->w2 : c->d_w2).as<cplx>();
+    cplx *hd = (ws ? ws->h : c->d_h).as<cplx>(), *part = (ws ? ws->part : c->d_part).as<cplx>();
+    DBuf &dnrm = ws ? ws->nrm : c->d_nrm;
+    cplx *&h_pin = ws ? ws->h_pin : c->h_pin;
+    int &h_pin_n = ws ? ws->h_pin_n : c->h_pin_n;
     const size_t ld = (size_t)c->Np;
     double bn = nrm(b);
     if(bn == 0.0) {
@@ -526,22 +582,29 @@ struct Gmres {
       *res = 0.0;
       return 0;
     }
-    if(c->h_pin_n < 2 * (m + 2) + 1) {
-      if(c->h_pin) (void)hipHostFree(c->h_pin);
-      c->h_pin = nullptr;
-      c->h_pin_n = 0;
-      HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&c->h_pin), sizeof(cplx) * (2 * (m + 2) + 1)));
-      c->h_pin_n = 2 * (m + 2) + 1;
+    if(h_pin_n < 2 * (m + 2) + 1) {
+      if(h_pin) (void)hipHostFree(h_pin);
+      h_pin = nullptr;
+      h_pin_n = 0;
+      HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&h_pin), sizeof(cplx) * (2 * (m + 2) + 1)));
+      h_pin_n = 2 * (m + 2) + 1;
     }
-    cplx *hpin = c->h_pin;
+    cplx *hpin = h_pin;
     int it = 0, stall = 0;
     double prev = 0.0;
     bool tiny_step = false;
     std::vector<std::complex<double>> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m);
+    int cycles = 0;
     while(true) {
-      matvec(x, r);
-      omx::vsub(r, b, r, N, s);
-      double beta = nrm(r);
+      if(one_cycle && cycles == 1) break;
+      if(one_cycle) {  // x = 0: r = b
+        omx::vcopy(r, b, N, s);
+      } else {
+        matvec(x, r);
+        omx::vsub(r, b, r, N, s);
+      }
+      ++cycles;
+      double beta = one_cycle ? bn : nrm(r);
       *res = beta / bn;
       if(*res <= tol || it >= maxit || tiny_step) break;
       // stagnation at the FP64 floor: a restart cycle without halving once the
@@ -574,8 +637,8 @@ struct Gmres {
           omx::axpy_multi(V, ld, j + 1, hp, w, N, s);
           HIPCHK(hipMemcpyAsync(hpin + (size_t)pass * (m + 2), hp, sizeof(cplx) * (j + 1), hipMemcpyDeviceToHost, s));
         }
-        omx::nrm2sq(w, N, c->d_nrm.as<double>() + 1, c->d_nrm.as<double>(), s);
-        HIPCHK(hipMemcpyAsync(hpin + 2 * (m + 2), c->d_nrm.p, sizeof(double), hipMemcpyDeviceToHost, s));
+        omx::nrm2sq(w, N, dnrm.as<double>() + 1, dnrm.as<double>(), s);
+        HIPCHK(hipMemcpyAsync(hpin + 2 * (m + 2), dnrm.p, sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         const cplx *h1 = hpin, *h2 = hpin + (m + 2);
         double hn = std::sqrt(reinterpret_cast<const double *>(hpin + 2 * (m + 2))[0]);
@@ -622,7 +685,7 @@ struct Gmres {
       const cplx *dx = z;
       omx::vaxpby(x, omx::mk(1.0), dx, omx::mk(1.0), N, s);
       // a correction below the FP64 resolution of x ends the refinement
-      if(nrm(dx) <= 1e-16 * nrm(x)) tiny_step = true;
+      if(!one_cycle && nrm(dx) <= 1e-16 * nrm(x)) tiny_step = true;
     }
     return it;
   }
@@ -703,6 +766,29 @@ void factor_enqueue(omx_context *c, bool shf, int pv) {
   (shf ? c->factored_sh : c->factored) = true;
 }
 
+// The target may sit below the FP64 residual floor of an ill-conditioned
+// system: a solution whose normwise backward error ||b - A x|| / (||A||_F ||x||
+// + ||b||) is within 1000 units of FP64 roundoff is accepted (the relative
+// residual's floor is u ||A|| ||x|| / ||b||, far above u when x is large).  The
+// bound is a constant: it does not grow with tol, so a caller's loose tol
+// never lets an FP32-quality answer through.
+bool backward_stable(omx_context *c, Gmres &gm, const cplx *A64, const cplx *rhs, const cplx *x, double res,
+                     hipStream_t st) {
+  const size_t Np = (size_t)c->Np;
+  const double bn = gm.nrm(rhs), xn = gm.nrm(x);
+  double a2 = 0.0;
+  const size_t n2 = Np * Np, chunk = (size_t)1 << 28;
+  for(size_t off = 0; off < n2; off += chunk) {
+    omx::nrm2sq(A64 + off, (int)std::min(chunk, n2 - off), c->d_nrm.as<double>() + 1, c->d_nrm.as<double>(), st);
+    double v = 0.0;
+    HIPCHK(hipMemcpyAsync(&v, c->d_nrm.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    a2 += v;
+  }
+  const double eta = (res * bn) / (std::sqrt(a2) * xn + bn);
+  return eta <= kBackwardErrorBound;
+}
+
 // Solves (S or V) x = rhs with the context's solver kind, on the system's
 // stream (a factorisation already enqueued by factor_enqueue is used).
 void solve_system(omx_context *c, bool shf, const cplx *rhs, cplx *x, int *iters, double *res) {
@@ -718,14 +804,44 @@ void solve_system(omx_context *c, bool shf, const cplx *rhs, cplx *x, int *iters
     gm.op = [c, shf, st](const cplx *in, cplx *out) { aca_apply(c, shf, in, out, st); };
     gm.op_bytes = c->aca[shf ? 1 : 0].bytes;
   }
+  // mixed Krylov: the preconditioner is one GMRES cycle on the FP32 copy
+  Gmres inner{c, nullptr, nullptr, Np, st};
+  if(c->kind == OMX_SOLVER_GMRES32_IR) {
+    float2 *A32w = (shf ? c->d_V32 : c->d_S32).as<float2>();
+    bool &dirty = shf ? c->dirty32_sh : c->dirty32;
+    if(dirty) omx::demote(A32w, A64, (size_t)Np * lda, st);  // an earlier factorisation overwrote it
+    dirty = false;
+    const float2 *A32c = A32w;
+    cplx *gpart = c->ws_in.gpart.as<cplx>();
+    inner.op = [A32c, lda, Np, gpart, st](const cplx *in, cplx *out) {
+      omx::gemv32(A32c, lda, Np, in, out, gpart, st);
+    };
+    inner.op_bytes = 8.0 * Np * (double)Np;
+    inner.ws = &c->ws_in;
+    inner.one_cycle = true;
+    gm.prec = [&inner, Np, st](const cplx *in, cplx *out) {
+      HIPCHK(hipMemsetAsync(out, 0, sizeof(cplx) * Np, st));
+      double r = 0.0;
+      inner.solve(in, out, kInnerTol, kInnerSteps, kInnerSteps, &r);
+    };
+  }
   if(!is_lu(c->kind)) {
-    HIPCHK(hipMemsetAsync(status.p, 0, 16, st));
-    HIPCHK(hipMemsetAsync(x, 0, sizeof(cplx) * Np, st));
-    *iters = gm.solve(rhs, x, c->tol, c->maxit, c->restart, res);
-    check_status(c, shf);
-    if(!(*res <= 1e3 * c->tol))
-      throw OmxError(OMX_ERR_NOCONV, "GMRES stopped at relative residual " + std::to_string(*res));
-    return;
+    const bool try_first = c->kind_req == OMX_SOLVER_AUTO;
+    try {
+      HIPCHK(hipMemsetAsync(status.p, 0, 16, st));
+      HIPCHK(hipMemsetAsync(x, 0, sizeof(cplx) * Np, st));
+      const int maxit =
+          try_first ? std::min(c->maxit, c->auto_budget > 0 ? c->auto_budget : auto_krylov_steps(c->N)) : c->maxit;
+      *iters = gm.solve(rhs, x, c->tol, maxit, c->restart, res);
+      check_status(c, shf);
+      if(!(*res <= 1e3 * c->tol) && (c->kind == OMX_SOLVER_ACA_GMRES || !backward_stable(c, gm, A64, rhs, x, *res, st)))
+        throw OmxError(OMX_ERR_NOCONV, "GMRES stopped at relative residual " + std::to_string(*res));
+      return;
+    } catch(OmxError &e) {
+      if(!(try_first && e.code == OMX_ERR_NOCONV)) throw;
+      c->kind = OMX_SOLVER_LU32_IR;  // the same S and S32: on to the mixed LU
+      ++c->fallbacks;
+    }
   }
   const bool f64 = c->kind == OMX_SOLVER_LU64;
   cplx *F64 = (shf ? c->d_F64sh : c->d_F64).as<cplx>();
@@ -764,22 +880,9 @@ void solve_system(omx_context *c, bool shf, const cplx *rhs, cplx *x, int *iters
       // floor is u ||A|| ||x|| / ||b||, far above u when x is large).  That
       // bound is a constant: it does not grow with tol, so a caller's loose
       // tol never lets an FP32-quality answer through.
-      if(!(*res <= 1e3 * c->tol)) {
-        const double bn = gm.nrm(rhs), xn = gm.nrm(x);
-        double a2 = 0.0;
-        const size_t n2 = (size_t)Np * lda, chunk = (size_t)1 << 28;
-        for(size_t off = 0; off < n2; off += chunk) {
-          omx::nrm2sq(A64 + off, (int)std::min(chunk, n2 - off), c->d_nrm.as<double>() + 1, c->d_nrm.as<double>(), st);
-          double v = 0.0;
-          HIPCHK(hipMemcpyAsync(&v, c->d_nrm.p, sizeof(double), hipMemcpyDeviceToHost, st));
-          HIPCHK(hipStreamSynchronize(st));
-          a2 += v;
-        }
-        const double eta = (*res * bn) / (std::sqrt(a2) * xn + bn);
-        if(!(eta <= kBackwardErrorBound))
-          throw OmxError(OMX_ERR_NOCONV, "refinement stopped at relative residual " + std::to_string(*res) +
-                                             ", backward error " + std::to_string(eta));
-      }
+      if(!(*res <= 1e3 * c->tol) && !backward_stable(c, gm, A64, rhs, x, *res, st))
+        throw OmxError(OMX_ERR_NOCONV, "refinement stopped at relative residual " + std::to_string(*res) +
+                                           ", backward error above " + std::to_string(kBackwardErrorBound));
       return;
     } catch(OmxError &e) {
       if(last || (e.code != OMX_ERR_SINGULAR && e.code != OMX_ERR_NOCONV)) throw;
@@ -1386,6 +1489,7 @@ void omx_destroy(omx_context *c) {
   for(DBuf *b : sbufs) b->release(&c->total);
   c->aca[0].release(&c->total);
   c->aca[1].release(&c->total);
+  c->ws_in.release(&c->total);
   DBuf *abufs[] = {&c->d_acaA, &c->d_acaU, &c->d_acaV, &c->d_acaR, &c->d_acaOff, &c->d_plist};
   for(DBuf *b : abufs) b->release(&c->total);
   for(auto &e : c->ev)
@@ -1471,13 +1575,20 @@ int omx_set_excitation(omx_context *c, double wavelength, double theta, double p
 
 int omx_set_solver(omx_context *c, int kind, double tol, int maxit, int restart) {
   return guard([&] {
-    if(!c || kind < 0 || kind > 3 || !(tol > 0.0) || maxit < 1 || restart < 1 || restart > 256)
+    if(!c || kind < 0 || kind > 5 || !(tol > 0.0) || maxit < 1 || restart < 1 || restart > 256)
       throw OmxError(OMX_ERR_ARG, "invalid solver settings");
-    c->kind = kind;
+    c->kind = c->kind_req = kind;
     c->tol = tol;
     c->maxit = maxit;
     c->restart = restart;
     c->updated = c->solved = false;
+  });
+}
+
+int omx_set_auto_budget(omx_context *c, int steps) {
+  return guard([&] {
+    if(!c || steps < 0) throw OmxError(OMX_ERR_ARG, "invalid budget");
+    c->auto_budget = steps;
   });
 }
 
@@ -1532,6 +1643,8 @@ int omx_update(omx_context *c) {
   return guard([&] {
     if(!c || !c->have_geom || !c->have_exc) throw OmxError(OMX_ERR_STATE, "geometry and excitation must be set");
     HIPCHK(hipSetDevice(c->device));
+    if(c->kind_req == OMX_SOLVER_AUTO)
+      c->kind = c->N >= kAutoKrylovN ? OMX_SOLVER_GMRES32_IR : OMX_SOLVER_LU32_IR;
     alloc_all(c);
     alloc_matrices(c);
     const int P = c->P, nmax = c->nmax;
@@ -1947,6 +2060,7 @@ int omx_solve_sweep(omx_context *c, int nl, const double *lambdas, const double 
   return guard([&] {
     if(!c || !c->have_geom || !c->have_exc || nl < 1 || !lambdas || batch < 0)
       throw OmxError(OMX_ERR_ARG, "invalid sweep (geometry, excitation and wavelengths required)");
+    if(c->kind_req == OMX_SOLVER_AUTO) c->kind = OMX_SOLVER_LU32_IR;  // batched sweeps factor
     if(!is_lu(c->kind)) throw OmxError(OMX_ERR_ARG, "sweeps need an LU solver kind");
     for(int l = 0; l < nl; ++l)
       if(!(lambdas[l] > 0.0)) throw OmxError(OMX_ERR_ARG, "non-positive wavelength");
@@ -2091,6 +2205,7 @@ int omx_trim(omx_context *c) {
     for(DBuf *b : big) b->release(&c->total);
     c->aca[0].release(&c->total);
     c->aca[1].release(&c->total);
+    c->ws_in.release(&c->total);
     c->updated = c->solved = false;
     c->factored = c->factored_sh = false;
   });

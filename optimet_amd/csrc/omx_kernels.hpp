@@ -215,6 +215,8 @@ void lu_solve(const T *A, int Np, size_t lda, LuWork &w, const cplx *b, cplx *x,
 // ---------------------------------------------------------------- iterative
 void gemv64(const cplx *A, size_t lda, int N, const cplx *x, cplx *y, cplx *part, hipStream_t st, int nb = 1,
             size_t sA = 0, size_t sv = 0);
+// y = A x with A the complex-FP32 copy (FP64 vectors and sums); part as gemv64's
+void gemv32(const float2 *A, size_t lda, int N, const cplx *x, cplx *y, cplx *part, hipStream_t st);
 constexpr int kGemvSplitMax = 32;
 
 // level-1 helpers (omx_iter.hip); reductions return through small device arrays

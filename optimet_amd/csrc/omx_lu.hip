@@ -1635,9 +1635,11 @@ template void lu_solve<double2>(const double2 *, int, size_t, LuWork &, const cp
 template void lu_solve<float2>(const float2 *, int, size_t, LuWork &, const cplx *, cplx *, hipStream_t);
 
 // ------------------------------------------------------------------ gemv
-// y = A x for the FP64 matrix (residuals and GMRES), two-stage column split.
+// y = A x for the FP64 matrix (residuals and GMRES) or its complex-FP32 copy
+// (entries widened to FP64, FP64 vectors and sums), two-stage column split.
 constexpr int kGemvSplit = 32;
-__global__ __launch_bounds__(256) void gemv_part_kernel(const cplx *__restrict__ A, size_t lda, int N,
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_part_kernel(const T *__restrict__ A, size_t lda, int N,
                                                         const cplx *__restrict__ x, cplx *part, int chunk, size_t sA,
                                                         size_t sv) {
   A += blockIdx.z * sA;
@@ -1653,13 +1655,14 @@ __global__ __launch_bounds__(256) void gemv_part_kernel(const cplx *__restrict__
     for(int i = threadIdx.x; i < ce - cb; i += 256) xs[i] = x[cb + i];
     __syncthreads();
     if(r < N) {
-      const cplx *a = A + (size_t)cb * lda + r;
+      const T *a = A + (size_t)cb * lda + r;
       int n = ce - cb;
 #pragma unroll 8
       for(int c = 0; c < n; ++c) {
-        cplx v = a[(size_t)c * lda];
-        acc.x += v.x * xs[c].x - v.y * xs[c].y;
-        acc.y += v.x * xs[c].y + v.y * xs[c].x;
+        const T e = a[(size_t)c * lda];
+        const double vx = e.x, vy = e.y;
+        acc.x += vx * xs[c].x - vy * xs[c].y;
+        acc.y += vx * xs[c].y + vy * xs[c].x;
       }
     }
   }
@@ -1679,9 +1682,16 @@ __global__ void gemv_reduce_kernel(const cplx *part, int ns, int N, cplx *y, siz
 void gemv64(const cplx *A, size_t lda, int N, const cplx *x, cplx *y, cplx *part, hipStream_t st, int nb, size_t sA,
             size_t sv) {
   int chunk = (N + kGemvSplit - 1) / kGemvSplit;
-  hipLaunchKernelGGL(gemv_part_kernel, dim3((N + 255) / 256, kGemvSplit, nb), dim3(256), 0, st, A, lda, N, x, part,
-                     chunk, sA, sv);
+  hipLaunchKernelGGL(gemv_part_kernel<cplx>, dim3((N + 255) / 256, kGemvSplit, nb), dim3(256), 0, st, A, lda, N, x,
+                     part, chunk, sA, sv);
   hipLaunchKernelGGL(gemv_reduce_kernel, dim3((N + 255) / 256, 1, nb), dim3(256), 0, st, part, kGemvSplit, N, y, sv);
+}
+void gemv32(const float2 *A, size_t lda, int N, const cplx *x, cplx *y, cplx *part, hipStream_t st) {
+  int chunk = (N + kGemvSplit - 1) / kGemvSplit;
+  hipLaunchKernelGGL(gemv_part_kernel<float2>, dim3((N + 255) / 256, kGemvSplit, 1), dim3(256), 0, st, A, lda, N, x,
+                     part, chunk, (size_t)0, (size_t)0);
+  hipLaunchKernelGGL(gemv_reduce_kernel, dim3((N + 255) / 256, 1, 1), dim3(256), 0, st, part, kGemvSplit, N, y,
+                     (size_t)0);
 }
 
 }  // namespace omx

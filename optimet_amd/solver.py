@@ -15,11 +15,12 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboptimet_mi355x.so")
 
-LU64, LU32_IR, GMRES64, ACA_GMRES = 0, 1, 2, 3
+LU64, LU32_IR, GMRES64, ACA_GMRES, GMRES32_IR, AUTO = 0, 1, 2, 3, 4, 5
 PIVOT_BLOCK, PIVOT_PARTIAL = 0, 1
 PIVOTS = {"block": PIVOT_BLOCK, "partial": PIVOT_PARTIAL}
 GEMMS = {"bf16x3": 0, "fp32": 1}
-KINDS = {"lu64": LU64, "lu32ir": LU32_IR, "gmres64": GMRES64, "aca": ACA_GMRES}
+KINDS = {"lu64": LU64, "lu32ir": LU32_IR, "gmres64": GMRES64, "aca": ACA_GMRES, "gmres32ir": GMRES32_IR,
+         "auto": AUTO}
 # Gmres_Zcomp settings of the reference's ACA path: tol 1e-6, 240 steps per
 # cycle, 2 cycles (PreconditionedMatrixSolver.h:47-56)
 ACA_DEFAULTS = {"tol": 1e-6, "maxit": 480, "restart": 240}
@@ -32,7 +33,7 @@ EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "o
            "omx_cross_sections", "omx_scattering_size", "omx_get_stats", "omx_read_matrix", "omx_read_vector",
            "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_set_lu32_gemm", "omx_solve_sweep", "omx_mem_info", "omx_trim",
            "omx_dense_solve_noalloc", "omx_read_matrix_block", "omx_absorption_cross_section_sh",
-           "omx_fields", "omx_aca_info", "omx_aca_block"]
+           "omx_fields", "omx_aca_info", "omx_aca_block", "omx_set_auto_budget"]
 
 
 class OmxError(RuntimeError):
@@ -98,6 +99,7 @@ def load_library(path=LIB_PATH):
     L.omx_fields.argtypes = [P, I, D, D, D, D, D]
     L.omx_aca_info.argtypes = [P, I, ctypes.POINTER(ctypes.c_int), D]
     L.omx_aca_block.argtypes = [P, I, I, I, ctypes.POINTER(ctypes.c_int), D, D]
+    L.omx_set_auto_budget.argtypes = [P, I]
     _lib = L
     return L
 
@@ -153,6 +155,10 @@ class Context:
         """Release the matrix-sized device buffers."""
         _check(self.L.omx_trim(self.h))
 
+    def set_auto_budget(self, steps):
+        """Outer Krylov steps the "auto" kind tries before the LU (0: the cost model)."""
+        _check(self.L.omx_set_auto_budget(self.h, int(steps)))
+
     def set_pivoting(self, mode):
         """"block" (default) or "partial" (OMX_PIVOT_*)."""
         _check(self.L.omx_set_pivoting(self.h, PIVOTS[mode]))
@@ -191,9 +197,12 @@ class Solver:
     """MI355X backend behind the AbstractSolver interface (Solver.h:62-144).
 
     kind: "lu64" (FP64 LU), "lu32ir" (complex-FP32 MFMA LU + FP64 GMRES-IR),
-    "gmres64" (GMRES on the FP64 matrix) or "aca" (<ACA compression="yes"/>:
-    the admissible blocks ACA-compressed, GMRES on the compressed operator;
-    tol, maxit and restart default to the reference's 1e-6, 480 and 240).
+    "gmres64" (GMRES on the FP64 matrix), "gmres32ir" (FP64 flexible GMRES
+    preconditioned by GMRES cycles on the complex-FP32 copy), "auto" (by size:
+    "gmres32ir" from N = 16384 on, within the mixed LU's cost, else or on
+    failure "lu32ir") or "aca" (<ACA compression="yes"/>: the admissible
+    blocks ACA-compressed, GMRES on the compressed operator; tol, maxit and
+    restart default to the reference's 1e-6, 480 and 240).
     pivoting: "block" (pivots within each 64 x 64 diagonal block, FP64
     refinement, automatic fallback) or "partial" (whole-column partial
     pivoting; with "lu64" the plain FP64 LU).

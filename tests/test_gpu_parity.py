@@ -87,7 +87,8 @@ def test_sh_matrix(ctx, ora):
     assert rel(V, Vo) < 1e-12
 
 
-@pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL), ("gmres64", MIXED_TOL)])
+@pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL), ("gmres64", MIXED_TOL),
+                                      ("gmres32ir", MIXED_TOL)])
 @pytest.mark.parametrize("name", ["config1", "offaxis"])
 def test_fundamental_solve(ctx, ora, name, kind, tol):
     c = CASES[name]()
@@ -102,7 +103,7 @@ def test_fundamental_solve(ctx, ora, name, kind, tol):
         assert abs(cs[k] - ref[k]) <= tol * abs(ref[k]), k
 
 
-@pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL)])
+@pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL), ("gmres32ir", MIXED_TOL)])
 def test_second_harmonic_chain(ctx, ora, kind, tol):
     c = small_drude(P=4, nmax=3)
     ref = ora.solve_all(c)

@@ -1,0 +1,83 @@
+"""The automatic solver choice (OMX_SOLVER_AUTO) and the mixed Krylov path
+(OMX_SOLVER_GMRES32_IR) at a size where the choice matters.
+
+At N >= 16384 the automatic kind starts with the mixed Krylov; with an
+iteration budget too small to converge it must fall back to the mixed LU on
+the same matrices and return the LU's answer.  Every path is held to the
+mixed bar (1e-9) against the mixed LU of the same system, whose own parity
+with the oracle is tested at smaller sizes (tests/test_gpu_parity.py) and at
+config 4 by residual (tests/test_gpu_configs.py)."""
+import numpy as np
+import pytest
+
+from optimet_amd.problem import drude_cluster
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def big():
+    """171 Drude spheres at nmax 6: N = 16416 (a 4.3 GB FP64 matrix)."""
+    return drude_cluster(171, 6, 800e-9, 30, 60, 2000, 10, seed=77, sh=False)
+
+
+@pytest.fixture(scope="module")
+def reference(big):
+    from optimet_amd import solver
+    s = solver.Solver(big, kind="lu32ir")
+    s.update()
+    out = s.solve()
+    cs = s.cross_sections()
+    s.ctx.close()
+    return out, cs
+
+
+def run(c, kind, budget=0, **kw):
+    from optimet_amd import solver
+    s = solver.Solver(c, kind=kind, **kw)
+    s.ctx.set_auto_budget(budget)
+    s.update()
+    out = s.solve()
+    cs = s.cross_sections()
+    st = s.stats()
+    s.ctx.close()
+    return out, cs, st
+
+
+def test_auto_picks_the_mixed_krylov(big, reference):
+    print(f"auto: N {big.N}", flush=True)
+    out, cs, st = run(big, "auto")
+    assert st["pivot_fallbacks"] == 0 and st["matvec_calls"] > 0
+    assert rel(out["x_sca"], reference[0]["x_sca"]) < 1e-9
+    assert cs["cext"] == pytest.approx(reference[1]["cext"], rel=1e-9)
+
+
+def test_auto_falls_back_to_the_mixed_lu(big, reference):
+    """One outer step cannot reach FP64 accuracy: the LU takes over."""
+    print(f"auto with one outer step: N {big.N}", flush=True)
+    out, cs, st = run(big, "auto", budget=1)
+    assert st["pivot_fallbacks"] >= 1
+    assert rel(out["x_sca"], reference[0]["x_sca"]) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["gmres32ir", "gmres64"])
+def test_krylov_paths_at_size(big, reference, kind):
+    print(f"{kind}: N {big.N}", flush=True)
+    out, cs, st = run(big, kind)
+    assert rel(out["x_sca"], reference[0]["x_sca"]) < 1e-9
+    assert st["residual"] <= 1e-13 or kind == "gmres64"
+
+
+def test_auto_small_system_uses_the_lu():
+    """Below N = 16384 the automatic kind is the mixed LU: no Krylov products
+    beyond the refinement's, and the LU's answer to rounding."""
+    c = drude_cluster(8, 5, 700e-9, 40, 80, 600, 10, seed=11, sh=True)
+    out_a, cs_a, st_a = run(c, "auto")
+    out_l, cs_l, st_l = run(c, "lu32ir")
+    assert st_a["iterations"] == st_l["iterations"]
+    assert rel(out_a["x_sca"], out_l["x_sca"]) < 1e-14
+    assert rel(out_a["x_sca_sh"], out_l["x_sca_sh"]) < 1e-13
