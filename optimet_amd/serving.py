@@ -24,10 +24,20 @@ is redone alone with partial pivoting; the sweep releases the batch's matrix
 buffers before that, so the request then holds one system's footprint, which
 is at most the batch's.
 
+With kind "auto" a request of N >= 16384 unknowns is solved wavelength by
+wavelength with the automatic solver choice (the mixed Krylov within the
+mixed LU's cost, else the mixed LU on the same matrices), one system at a
+time; smaller requests use the batched mixed-LU sweeps.
+
+Results are cached by content: a wavelength of a cluster (geometry,
+materials, excitation) that an earlier request solved is answered from the
+cache without device work, and the server counts hits and misses.
+
 The scheduler is plain Python over the C ABI (ctypes releases the GIL during
 the calls); `solve_fn` can be replaced for CPU tests of the admission logic.
 """
 import collections
+import hashlib
 import threading
 import time
 
@@ -36,13 +46,21 @@ import numpy as np
 from . import solver as _solver
 
 
+KRYLOV_N = 16384  # "auto": requests from this order on are solved one wavelength at a time
+
+
 def footprint(cluster, batch, kind):
     """Device bytes of a batch of `batch` systems of this cluster: the FP64
     matrix (16 N^2) and, per solver kind, its FP32 copy (8 N^2) or FP64
     factors (16 N^2), for S and (with the second harmonic) V, plus the
-    per-system vectors, panel inverses and the two Krylov bases (restart 40)."""
+    per-system vectors, panel inverses and the two Krylov bases (restart 40);
+    "auto" is the mixed LU's footprint (the Krylov path holds the same
+    matrices plus a 31-vector inner basis)."""
     Np = -(-cluster.N // 64) * 64
-    per_matrix = Np * Np * (24 if kind == "lu32ir" else 32)
+    per_matrix = Np * Np * (32 if kind == "lu64" else 24)
+    if kind == "auto" and cluster.N >= KRYLOV_N:
+        batch = 1
+        per_matrix += Np * 16 * 32 // (2 if cluster.sh else 1)
     mats = 2 if cluster.sh else 1
     vectors = Np * (16 * 16 + 2 * 16 * 2 * 64 + 2 * 16 * 41 + 16 * 32)
     return batch * (mats * per_matrix + vectors)
@@ -54,6 +72,56 @@ class Request:
         self.arrival = float(arrival)
         self.cluster = cluster
         self.wavelengths = np.asarray(wavelengths, dtype=np.float64)
+
+
+def cluster_key(c):
+    """Content hash of everything a solve depends on except the wavelength."""
+    h = hashlib.sha1()
+    for a in (c.xyz, c.radius, c.eps_r, c.eps_r_sh, c.mu_r, c.ksippp, c.ksiparppar, c.gamma):
+        h.update(np.ascontiguousarray(a).tobytes())
+    h.update(repr((c.nmax, bool(c.sh), c.bg_eps_r, c.bg_mu_r, c.theta_k, c.phi_k, c.etheta, c.ephi,
+                   c.meta.get("drude"))).encode())
+    return h.hexdigest()
+
+
+class ResultCache:
+    """Per-wavelength results (x_sca, x_sca_sh, cross sections, iterations)
+    of solved clusters, keyed by content, least recently used dropped past
+    `max_bytes` of host memory."""
+
+    def __init__(self, max_bytes=4 << 30):
+        self.max_bytes = max_bytes
+        self.bytes = 0
+        self.hits = 0
+        self.misses = 0
+        self._d = collections.OrderedDict()
+        self._lock = threading.Lock()
+
+    def get(self, key):
+        with self._lock:
+            v = self._d.get(key)
+            if v is None:
+                self.misses += 1
+                return None
+            self._d.move_to_end(key)
+            self.hits += 1
+            return v
+
+    def put(self, key, value):
+        size = sum(a.nbytes for a in value.values() if isinstance(a, np.ndarray))
+        with self._lock:
+            if key in self._d:
+                return
+            self._d[key] = value
+            self.bytes += size
+            while self.bytes > self.max_bytes and self._d:
+                _, old = self._d.popitem(last=False)
+                self.bytes -= sum(a.nbytes for a in old.values() if isinstance(a, np.ndarray))
+
+    def stats(self):
+        n = self.hits + self.misses
+        return {"hits": self.hits, "misses": self.misses, "hit_rate": self.hits / n if n else 0.0,
+                "entries": len(self._d), "bytes": self.bytes}
 
 
 class Result:
@@ -77,8 +145,12 @@ class Server:
     bytes the admitted requests may hold (default: 60% of the free memory at
     start); batch_cap: most wavelengths per batched sweep."""
 
-    def __init__(self, slots=3, budget=None, kind="lu32ir", device=0, batch_cap=32, solve_fn=None):
+    def __init__(self, slots=3, budget=None, kind="lu32ir", device=0, batch_cap=32, solve_fn=None,
+                 cache_bytes=4 << 30, profile=False):
         self.kind = kind
+        self.cache = ResultCache(cache_bytes)
+        self.profile = profile  # per-launch HIP events; each request's counters kept in prof_log
+        self.prof_log = []
         self.slots = slots
         self.batch_cap = batch_cap
         self._solve_fn = solve_fn
@@ -108,13 +180,51 @@ class Server:
         return b, footprint(req.cluster, b, self.kind)
 
     def _solve(self, slot, req, batch):
-        if self._solve_fn is not None:
+        """The request's wavelengths not in the cache, solved; then every
+        wavelength's result from the cache.  Returns the sweep's layout:
+        x_sca, x_sca_sh (n_lambda, N), xs (n_lambda, 5), iterations."""
+        if self._solve_fn is not None:  # a stand-in for the whole request (CPU tests of the admission rules)
             return self._solve_fn(slot, req, batch)
+        key = cluster_key(req.cluster)
+        lams = req.wavelengths
+        cached = [self.cache.get((key, float(l))) for l in lams]
+        todo = np.array([l for l, v in zip(lams, cached) if v is None])
+        fresh = {}
+        if todo.size:
+            out = self._device_solve(slot, req, batch, todo)
+            for i, l in enumerate(todo):
+                fresh[float(l)] = {k: v[i] for k, v in out.items()}
+                self.cache.put((key, float(l)), fresh[float(l)])
+        per = [fresh[float(l)] if v is None else v for l, v in zip(lams, cached)]
+        return {k: np.stack([p[k] for p in per]) for k in per[0]}
+
+    def _device_solve(self, slot, req, batch, lams):
         ctx = self._contexts[slot]
-        s = _solver.Solver(req.cluster, kind=self.kind, context=ctx)
+        c = req.cluster
+        if self.profile:
+            ctx.set_profile(True)  # also resets the context's counters
         try:
-            return s.sweep(req.wavelengths, batch=batch, outputs=("x_sca", "x_sca_sh"))
+            if self.kind == "auto" and c.N >= KRYLOV_N:  # one wavelength at a time, automatic choice
+                s = _solver.Solver(c.with_wavelength(lams[0]), kind="auto", context=ctx)
+                outs = {"x_sca": [], "x_sca_sh": [], "xs": [], "iterations": []}
+                for lam in lams:
+                    s.set_cluster(c.with_wavelength(lam))
+                    s.update()
+                    o = s.solve()
+                    cs = s.cross_sections()
+                    st = s.stats()
+                    outs["x_sca"].append(o["x_sca"])
+                    if c.sh:
+                        outs["x_sca_sh"].append(o["x_sca_sh"])
+                    outs["xs"].append([cs["cext"], cs["csca"], cs["cabs"], cs["csca_sh"], cs["cabs_sh"]])
+                    outs["iterations"].append([st["iterations"], st["iterations_sh"]])
+                return {k: np.array(v) for k, v in outs.items() if len(v)}
+            kind = "lu32ir" if self.kind == "auto" else self.kind
+            s = _solver.Solver(c, kind=kind, context=ctx)
+            return s.sweep(lams, batch=batch, outputs=("x_sca", "x_sca_sh"))
         finally:
+            if self.profile:
+                self.prof_log.append((req.id, ctx.stats()))
             ctx.trim()
 
     def _worker(self, slot, req, res):

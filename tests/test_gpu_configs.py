@@ -241,8 +241,9 @@ def _config5_sample():
 
 
 def test_config5_served_sample(ora):
-    """Requests served concurrently (three slots, footprint admission) give
-    the oracle's answer: full chain where N <= 4000, residuals of x_sca on S
+    """Requests served concurrently (three slots, footprint admission, the
+    automatic choice per request) give the oracle's answer: full chain where
+    N <= 4000 (batched mixed-LU sweeps), residuals of x_sca on S
     and of x_sca_sh on V (with the oracle's SH source from the served x_sca)
     otherwise."""
     from optimet_amd.serving import Request, Server
@@ -250,7 +251,7 @@ def test_config5_served_sample(ora):
     assert sorted({c.nmax for _, c, _ in sample}) == list(range(4, 11))
     assert max(c.nobj for _, c, _ in sample) == 128
     reqs = [Request(rid, 0.0, c, lams) for rid, c, lams in sample]
-    srv = Server(slots=3, kind="lu32ir")
+    srv = Server(slots=3, kind="auto")  # the large requests go through the mixed Krylov
     try:
         results = srv.run(reqs, realtime=False)
     finally:

@@ -144,3 +144,35 @@ def test_small_request_backfills_while_head_waits_for_memory():
     assert all(r.error is None for r in res)
     # the small request started before the second big one
     assert res[2].admitted < res[1].admitted
+
+
+def test_result_cache_answers_repeated_wavelengths():
+    """A wavelength of a cluster solved before is answered from the cache:
+    only the new wavelengths reach the solver, results come back in request
+    order, and the hit rate counts per wavelength.  A cluster that differs in
+    any input (here one permittivity) is a different key."""
+    s = Server(slots=1, budget=1 << 40, solve_fn=lambda *a: None)
+    s._solve_fn = None
+    seen = []
+
+    def device(slot, req, batch, lams):
+        seen.append(list(lams))
+        n = lams.size
+        return {"x_sca": np.outer(lams, np.arange(3.0)) + 1j, "xs": np.tile(lams[:, None], (1, 5)),
+                "iterations": np.zeros((n, 2), np.int32)}
+
+    s._device_solve = device
+    a = _req(0, 0.0, nl=3)
+    b = Request(1, 0.0, a.cluster, np.array([a.wavelengths[2], 900e-9, a.wavelengths[0]]))
+    res = s.run([a, b], realtime=False)
+    assert all(r.error is None for r in res), [r.error for r in res]
+    assert seen == [list(a.wavelengths), [900e-9]]
+    np.testing.assert_array_equal(res[1].out["xs"][:, 0], b.wavelengths)
+    np.testing.assert_array_equal(res[1].out["x_sca"][2], res[0].out["x_sca"][0])
+    st = s.cache.stats()
+    assert (st["hits"], st["misses"]) == (2, 4)
+    c = _req(0, 0.0, nl=3)
+    c.cluster.eps_r = c.cluster.eps_r.copy()
+    c.cluster.eps_r[0] += 1e-12
+    s.run([Request(2, 0.0, c.cluster, a.wavelengths)], realtime=False)
+    assert seen[-1] == list(a.wavelengths)
