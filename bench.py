@@ -72,7 +72,7 @@ def pct(v, q):
     return float(np.percentile(np.asarray(v), q)) if v else float("nan")
 
 
-def run_path(cluster, kind, steps, warmup, profile):
+def run_path(cluster, kind, steps, warmup, profile, pivoting="block"):
     """Times `steps` unprofiled steps (the value and latencies), then, with
     `profile`, runs one more step with per-launch HIP events on (which
     serialises the omega and 2 omega factorisations) for the stage and GEMM
@@ -80,7 +80,7 @@ def run_path(cluster, kind, steps, warmup, profile):
     from optimet_amd import solver
 
     ctx = solver.Context()
-    s = solver.Solver(cluster, kind=kind, context=ctx)
+    s = solver.Solver(cluster, kind=kind, context=ctx, pivoting=pivoting)
     for _ in range(warmup):
         s.update()
         s.solve()
@@ -440,6 +440,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 LU comparison path")
+    ap.add_argument("--pivoting", choices=("block", "partial"), default="block",
+                    help="config2: pivot search of the mixed path's factorisations")
     ap.add_argument("--legs", default="auto,gmres64,gmres32ir,lu32ir,lu64,aca",
                     help="config4: solver paths to run (a subset prints their figures only)")
     ap.add_argument("--workload", choices=("config2", "config3", "config4", "config5"), default="config2",
@@ -483,7 +485,7 @@ def main():
     from optimet_amd.problem import config2
 
     c = config2()
-    mixed = run_path(c, "lu32ir", args.steps, args.warmup, True)
+    mixed = run_path(c, "lu32ir", args.steps, args.warmup, True, pivoting=args.pivoting)
     fp64 = None if args.no_fp64 else run_path(c, "lu64", args.steps, args.warmup, True)
 
     ms = mixed["stats"]

@@ -150,7 +150,7 @@ struct omx_context {
     DBuf S64, S32, V64, V32, F64, F64sh, eps, eps_sh, ov, incloc, Q, w, x, xint, xsh, xintsh, K, K1, cross, abscoef,
         abssh;
     DBuf perm, perm2, flags, flags2, dinv, dinv2, map, status, basis, basisZ, r, w1, w2, gpart, part, hd, nrm, sc,
-        act, kz, omz;
+        act, kz, omz, sys, map2, status2;
   } sw;
   // ACA-compressed operators S (0) and V (1) of OMX_SOLVER_ACA_GMRES
   struct AcaMat {
@@ -905,7 +905,36 @@ struct GmresB {
   int Np, nb;
   std::function<void(const cplx *, cplx *)> prec;
   omx_context::Sweep &w;
+  omx::LuWork *lw = nullptr;  // the preconditioner's factorisation: its solves may take a subset
   size_t sv() const { return (size_t)Np; }
+  // The systems with mask[z] set: their count, and (when some are left out)
+  // their indices in w.sys, so the matrix-sized work runs on those only.
+  // Converged systems thus drop out of the lockstep batch's products and
+  // triangular solves; their vectors are left as they are and ignored.
+  int subset(const std::vector<int> &mask, const int **sys) {
+    std::vector<int> idx;
+    for(int z = 0; z < nb; ++z)
+      if(mask[z]) idx.push_back(z);
+    *sys = nullptr;
+    if((int)idx.size() == nb || idx.empty()) return (int)idx.size();
+    HIPCHK(hipMemcpyAsync(w.sys.p, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, c->st));
+    *sys = w.sys.as<int>();
+    return (int)idx.size();
+  }
+  void precond(const cplx *in, cplx *out, const std::vector<int> &mask) {
+    if(!lw) {
+      prec(in, out);
+      return;
+    }
+    const int *sys;
+    const int n = subset(mask, &sys);
+    if(n == 0) return;
+    lw->sys = sys;
+    lw->nsys = n;
+    prec(in, out);
+    lw->sys = nullptr;
+    lw->nsys = 0;
+  }
   std::vector<double> nrms(const cplx *x) {
     omx::nrm2sq_b(x, sv(), Np, nb, w.nrm.as<double>() + nb, w.nrm.as<double>(), c->st);
     std::vector<double> v(nb);
@@ -914,11 +943,14 @@ struct GmresB {
     for(double &e : v) e = std::sqrt(e);
     return v;
   }
-  void matvec(const cplx *x, cplx *y) {
+  void matvec(const cplx *x, cplx *y, const std::vector<int> &mask) {
+    const int *sys;
+    const int n = subset(mask, &sys);
+    if(n == 0) return;
     if(c->profile) HIPCHK(hipEventRecord(c->ev[4], c->st));
-    omx::gemv64(A, (size_t)Np, Np, x, y, w.gpart.as<cplx>(), c->st, nb, sA, sv());
+    omx::gemv64(A, (size_t)Np, Np, x, y, w.gpart.as<cplx>(), c->st, nb, sA, sv(), sys, n);
     ++c->mv_calls;
-    c->mv_bytes += 16.0 * Np * (double)Np * nb;
+    c->mv_bytes += 16.0 * Np * (double)Np * n;
     if(c->profile) {
       HIPCHK(hipEventRecord(c->ev[5], c->st));
       HIPCHK(hipEventSynchronize(c->ev[5]));
@@ -971,7 +1003,7 @@ struct GmresB {
     std::vector<int> jz(nb);
     int it = 0;
     while(true) {
-      matvec(x, r);
+      matvec(x, r, active);
       omx::vsub_b(r, b, r, Np, nb, sv(), c->st);
       const std::vector<double> beta = nrms(r);
       bool any = false;
@@ -1010,8 +1042,8 @@ struct GmresB {
       for(int j = 0; j < m && it < maxit; ++j, ++it) {
         const cplx *vj = V + (size_t)j * ld;
         cplx *zj = Zb + (size_t)j * ld;
-        prec(vj, zj);
-        matvec(zj, wv);
+        precond(vj, zj, live);
+        matvec(zj, wv, live);
         std::vector<cplx> h1((size_t)nb * sh), h2((size_t)nb * sh);
         for(int pass = 0; pass < 2; ++pass) {
           omx::dotc_multi_b(V, ld, sV, j + 1, wv, sv(), Np, nb, part, hd, sh, c->st);
@@ -1124,6 +1156,11 @@ void sweep_alloc(omx_context *c, int nb, bool f64, bool sh) {
   for(DBuf *b : {&w.dinv, &w.dinv2}) b->ensure(sizeof(cplx) * omx::dinv_elems(Np) * nb, T);
   w.map.ensure(sizeof(int) * 3 * (Np + 2 * omx::kNB) * nb, T);
   w.status.ensure(sizeof(int) * 4 * nb, T);
+  w.sys.ensure(sizeof(int) * nb, T);
+  if(sh) {  // V's factorisation runs beside S's: maps and status words of its own
+    w.map2.ensure(sizeof(int) * 3 * (Np + 2 * omx::kNB) * nb, T);
+    w.status2.ensure(sizeof(int) * 4 * nb, T);
+  }
   w.basis.ensure(sizeof(cplx) * Np * (m + 1) * nb, T);
   w.basisZ.ensure(sizeof(cplx) * Np * (m + 1) * nb, T);
   w.gpart.ensure(sizeof(cplx) * Np * omx::kGemvSplitMax * nb, T);
@@ -1145,16 +1182,19 @@ omx::LuWork sweep_lu(omx_context *c, int nb, bool second) {
   lw.flags = (second ? w.flags2 : w.flags).as<int>();
   lw.dinv = (second ? w.dinv2 : w.dinv).p;
   const int ms = Np + 2 * omx::kNB;
-  lw.mdst = w.map.as<int>();
-  lw.msrc = w.map.as<int>() + (size_t)ms * nb;
-  lw.mcnt = w.map.as<int>() + 2 * (size_t)ms * nb;
-  lw.status = w.status.as<int>();
-  lw.wlu = c->d_wlu.p;
-  lw.aux = c->aux;
-  lw.ev_next = c->lu_ev[0];
-  lw.ev_piv = c->lu_ev[1];
-  lw.ev_bulk = c->lu_ev[2];
-  lw.ev_pan = c->lu_ev[3];
+  int *map = (second ? w.map2 : w.map).as<int>();
+  lw.mdst = map;
+  lw.msrc = map + (size_t)ms * nb;
+  lw.mcnt = map + 2 * (size_t)ms * nb;
+  lw.status = (second ? w.status2 : w.status).as<int>();
+  lw.wlu = c->d_wlu.p ? c->d_wlu.as<cplx>() + (second ? 3 * omx::kNB * omx::kNB : 0) : nullptr;
+  // V's factorisation runs on the second stream pair, beside S's work
+  lw.aux = second ? c->aux2 : c->aux;
+  const hipEvent_t *ev = second ? c->lu_ev2 : c->lu_ev;
+  lw.ev_next = ev[0];
+  lw.ev_piv = ev[1];
+  lw.ev_bulk = ev[2];
+  lw.ev_pan = ev[3];
   lw.gmax = 0;
   lw.epoch = 0;
   lw.pivot = omx::kPivotBlock;
@@ -1191,6 +1231,11 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
   HIPCHK(hipMemsetAsync(w.flags.p, 0, w.flags.bytes, st));
   HIPCHK(hipMemsetAsync(w.flags2.p, 0, w.flags2.bytes, st));
   HIPCHK(hipMemsetAsync(w.status.p, 0, w.status.bytes, st));
+  if(sh) HIPCHK(hipMemsetAsync(w.status2.p, 0, w.status2.bytes, st));
+  // the lockstep refinement leaves the vectors of systems that dropped out
+  // as they are: start them finite
+  HIPCHK(hipMemsetAsync(w.basisZ.p, 0, w.basisZ.bytes, st));
+  HIPCHK(hipMemsetAsync(w.w1.p, 0, w.w1.bytes, st));
   HIPCHK(hipMemsetAsync(w.Q.p, 0, w.Q.bytes, st));
   HIPCHK(hipMemsetAsync(c->d_status.p, 0, 16, st));
   const cplx bge = kEps0 * c->bg_eps_r, bgm = kMu0 * c->bg_mu_r;
@@ -1246,9 +1291,10 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
     if(!f64) omx::run_pad32(cV32, N, Np, lda, st, nb, n2);
   }
   HIPCHK(hipGetLastError());
-  auto status_scan = [&](std::vector<int> &flag) {
+  auto status_scan = [&](std::vector<int> &flag, bool second) {
     std::vector<int> h(4 * nb);
-    HIPCHK(hipMemcpyAsync(h.data(), w.status.p, sizeof(int) * 4 * nb, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(h.data(), (second ? w.status2 : w.status).p, sizeof(int) * 4 * nb, hipMemcpyDeviceToHost,
+                          st));
     int g[2];
     HIPCHK(hipMemcpyAsync(g, c->d_status.p, sizeof(g), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -1256,24 +1302,42 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
     for(int z = 0; z < nb; ++z)
       if(h[4 * z]) flag[z] = 1;
   };
-  // solves (S or V) for every system; returns per-system convergence flags
-  auto solve_all = [&](bool second, const cplx *rhs, cplx *x) {
+  // factorisation of S (V) for every system, enqueued on stream fst
+  auto factor = [&](bool second, omx::LuWork &lw, hipStream_t fst) -> void * {
     cplx *A64 = second ? cV64 : cS64;
-    omx::LuWork lw = sweep_lu(c, nb, second);
-    void *F = nullptr;
+    void *F;
     int rc;
     if(f64) {
       F = (second ? w.F64sh : w.F64).p;
-      HIPCHK(hipMemcpyAsync(F, A64, sizeof(cplx) * n2 * nb, hipMemcpyDeviceToDevice, st));
-      rc = omx::lu_factor<double2>(static_cast<double2 *>(F), Np, lda, lw, st, nullptr);
+      HIPCHK(hipMemcpyAsync(F, A64, sizeof(cplx) * n2 * nb, hipMemcpyDeviceToDevice, fst));
+      rc = omx::lu_factor<double2>(static_cast<double2 *>(F), Np, lda, lw, fst, nullptr);
     } else {
       F = (second ? w.V32 : w.S32).p;
-      rc = omx::lu_factor<float2>(static_cast<float2 *>(F), Np, lda, lw, st, nullptr);
+      rc = omx::lu_factor<float2>(static_cast<float2 *>(F), Np, lda, lw, fst, nullptr);
     }
     if(rc) throw OmxError(OMX_ERR_DEVICE, "batched lu_factor launch failed (" + std::to_string(rc) + ")");
-    status_scan(bad);
+    return F;
+  };
+  // V does not depend on the fundamental solution: its factorisation starts
+  // on the second stream pair right after the assembly and runs beside S's
+  // factorisation and refinement; V's refinement waits for it
+  omx::LuWork lwS = sweep_lu(c, nb, false), lwV{};
+  void *FV = nullptr;
+  if(sh) {
+    lwV = sweep_lu(c, nb, true);
+    HIPCHK(hipEventRecord(c->xev[0], st));
+    HIPCHK(hipStreamWaitEvent(c->st2, c->xev[0], 0));
+    FV = factor(true, lwV, c->st2);
+    HIPCHK(hipEventRecord(c->xev[1], c->st2));
+  }
+  void *FS = factor(false, lwS, st);
+  // refinement of every system of S (V); per-system convergence flags
+  auto refine = [&](bool second, omx::LuWork &lw, void *F, const cplx *rhs, cplx *x) {
+    status_scan(bad, second);
+    cplx *A64 = second ? cV64 : cS64;
     omx::LuWork *lwp = &lw;
     GmresB gm{c, A64, n2, Np, nb, nullptr, w};
+    gm.lw = lwp;
     if(f64)
       gm.prec = [c, F, lwp, Np, lda](const cplx *in, cplx *out) {
         omx::lu_solve<double2>(static_cast<const double2 *>(F), Np, lda, *lwp, in, out, c->st);
@@ -1290,14 +1354,14 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
     // one-at-a-time path (partial pivoting) instead of holding the batch
     const int bmax = nb > 1 ? std::min(c->maxit, 4 * c->restart) : c->maxit;
     gm.solve(rhs, x, c->tol, bmax, c->restart, res, its);
-    status_scan(bad);
+    status_scan(bad, second);
     for(int z = 0; z < nb; ++z) {
       if(!(res[z] <= 1e3 * c->tol)) bad[z] = 1;
       iters[2 * z + (second ? 1 : 0)] = its[z];
       resid[2 * z + (second ? 1 : 0)] = res[z];
     }
   };
-  solve_all(false, w.Q.as<cplx>(), w.x.as<cplx>());
+  refine(false, lwS, FS, w.Q.as<cplx>(), w.x.as<cplx>());
   for(int z = 0; z < nb; ++z)
     omx::vmul_ov(w.xint.as<cplx>() + z * lda, w.x.as<cplx>() + z * lda, w.ov.as<cplx>() + z * ovs, omx::OV_IAUX, P,
                  nmax, nullptr, N, st);
@@ -1317,7 +1381,8 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
                      w.xint.as<cplx>() + z * lda, w.K.as<cplx>() + z * lda, w.K1.as<cplx>() + z * lda};
       omx::run_shsrc(sa, st);
     }
-    solve_all(true, w.K.as<cplx>(), w.xsh.as<cplx>());
+    HIPCHK(hipStreamWaitEvent(st, c->xev[1], 0));  // V's factorisation
+    refine(true, lwV, FV, w.K.as<cplx>(), w.xsh.as<cplx>());
     for(int z = 0; z < nb; ++z)
       omx::vmul_ov(w.xintsh.as<cplx>() + z * lda, w.xsh.as<cplx>() + z * lda, w.ov.as<cplx>() + z * ovs,
                    omx::OV_IAUXSH1, P, nmax, w.K1.as<cplx>() + z * lda, N, st);

@@ -181,6 +181,8 @@ struct LuWork {
   int pivot;            // kPivotBlock or kPivotPartial
   int nb = 1;           // systems in the batch (block pivoting only when > 1)
   BatchStrides bs;      // their strides; the pointers above are system 0's
+  const int *sys = nullptr;  // lu_solve on a subset: its nsys systems' indices (null: all nb)
+  int nsys = 0;
   bool split = true;             // FP32 factors: bulk updates on the split-bf16 MFMA
   hipEvent_t ev_mark = nullptr;  // when set: recorded on the bulk stream once the
   int mark_col = 0;              // factorisation has passed column mark_col, and
@@ -213,8 +215,9 @@ template <typename T>
 void lu_solve(const T *A, int Np, size_t lda, LuWork &w, const cplx *b, cplx *x, hipStream_t st);
 
 // ---------------------------------------------------------------- iterative
+// (sys non-null: only the nsys systems it lists)
 void gemv64(const cplx *A, size_t lda, int N, const cplx *x, cplx *y, cplx *part, hipStream_t st, int nb = 1,
-            size_t sA = 0, size_t sv = 0);
+            size_t sA = 0, size_t sv = 0, const int *sys = nullptr, int nsys = 0);
 // y = A x with A the complex-FP32 copy (FP64 vectors and sums); part as gemv64's
 void gemv32(const float2 *A, size_t lda, int N, const cplx *x, cplx *y, cplx *part, hipStream_t st);
 constexpr int kGemvSplitMax = 32;

@@ -1362,10 +1362,11 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
 }
 
 // ------------------------------------------------------------------ solves
-__global__ void gather_kernel(const cplx *b, const int *perm, cplx *y, int Np, BatchStrides bs) {
-  b += blockIdx.z * bs.vec;
-  y += blockIdx.z * bs.vec;
-  perm += (size_t)blockIdx.z * bs.perm;
+__global__ void gather_kernel(const cplx *b, const int *perm, cplx *y, int Np, BatchStrides bs, const int *sys) {
+  const size_t z = sys ? sys[blockIdx.z] : blockIdx.z;
+  b += z * bs.vec;
+  y += z * bs.vec;
+  perm += z * bs.perm;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i < Np) y[i] = b[perm[i]];
 }
@@ -1391,13 +1392,14 @@ __global__ void gather_kernel(const cplx *b, const int *perm, cplx *y, int Np, B
 template <typename T, bool LOWER, int SB, int PRE>
 __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size_t lda, int Np,
                                                    const T *__restrict__ dinv, cplx *x, int *flags, int epoch,
-                                                   int *status, BatchStrides bs) {
+                                                   int *status, BatchStrides bs, const int *sys) {
   constexpr int NB = kNB;
   constexpr int NI = SB * (SB - 1) / 2;
-  A += blockIdx.z * bs.a;
-  dinv += blockIdx.z * bs.inv;
-  x += blockIdx.z * bs.vec;
-  flags += (size_t)blockIdx.z * bs.flags;
+  const size_t zs = sys ? sys[blockIdx.z] : blockIdx.z;  // the system (a subset may be listed)
+  A += zs * bs.a;
+  dinv += zs * bs.inv;
+  x += zs * bs.vec;
+  flags += zs * bs.flags;
   const int nblk = Np / NB, nsb = (nblk + SB - 1) / SB;
   const int K = LOWER ? blockIdx.x : nsb - 1 - blockIdx.x;
   const int kb0 = K * SB, nb = min(SB, nblk - kb0);
@@ -1619,14 +1621,16 @@ void lu_solve(const T *A, int Np, size_t lda, LuWork &w, const cplx *b, cplx *x,
   const T *dinv = reinterpret_cast<const T *>(w.dinv);
   constexpr int SB = TrsvCfg<T>::SB;
   constexpr int PRE = TrsvCfg<T>::PRE;
-  hipLaunchKernelGGL(gather_kernel, dim3((Np + 255) / 256, 1, w.nb), dim3(256), 0, st, b, w.perm, x, Np, w.bs);
+  const int nz = w.sys ? w.nsys : w.nb;
+  if(nz < 1) return;
+  hipLaunchKernelGGL(gather_kernel, dim3((Np + 255) / 256, 1, nz), dim3(256), 0, st, b, w.perm, x, Np, w.bs, w.sys);
   const int nblk = Np / kNB, nsb = (nblk + SB - 1) / SB;
   int e1 = ++w.epoch;
-  hipLaunchKernelGGL((trsv_kernel<T, true, SB, PRE>), dim3(nsb, 1, w.nb), dim3(256), 0, st, A, lda, Np, dinv, x,
-                     w.flags, e1, w.status, w.bs);
+  hipLaunchKernelGGL((trsv_kernel<T, true, SB, PRE>), dim3(nsb, 1, nz), dim3(256), 0, st, A, lda, Np, dinv, x,
+                     w.flags, e1, w.status, w.bs, w.sys);
   int e2 = ++w.epoch;
-  hipLaunchKernelGGL((trsv_kernel<T, false, SB, PRE>), dim3(nsb, 1, w.nb), dim3(256), 0, st, A, lda, Np, dinv, x,
-                     w.flags, e2, w.status, w.bs);
+  hipLaunchKernelGGL((trsv_kernel<T, false, SB, PRE>), dim3(nsb, 1, nz), dim3(256), 0, st, A, lda, Np, dinv, x,
+                     w.flags, e2, w.status, w.bs, w.sys);
 }
 
 template int lu_factor<double2>(double2 *, int, size_t, LuWork &, hipStream_t, LuProf *);
@@ -1641,9 +1645,10 @@ constexpr int kGemvSplit = 32;
 template <typename T>
 __global__ __launch_bounds__(256) void gemv_part_kernel(const T *__restrict__ A, size_t lda, int N,
                                                         const cplx *__restrict__ x, cplx *part, int chunk, size_t sA,
-                                                        size_t sv) {
-  A += blockIdx.z * sA;
-  x += blockIdx.z * sv;
+                                                        size_t sv, const int *sys) {
+  const size_t zs = sys ? sys[blockIdx.z] : blockIdx.z;
+  A += zs * sA;
+  x += zs * sv;
   part += (size_t)blockIdx.z * gridDim.y * N;
   __shared__ cplx xs[512];
   const int r = blockIdx.x * 256 + threadIdx.x;
@@ -1668,9 +1673,9 @@ __global__ __launch_bounds__(256) void gemv_part_kernel(const T *__restrict__ A,
   }
   if(r < N) part[(size_t)blockIdx.y * N + r] = acc;
 }
-__global__ void gemv_reduce_kernel(const cplx *part, int ns, int N, cplx *y, size_t sv) {
+__global__ void gemv_reduce_kernel(const cplx *part, int ns, int N, cplx *y, size_t sv, const int *sys) {
   part += (size_t)blockIdx.z * ns * N;
-  y += blockIdx.z * sv;
+  y += (sys ? sys[blockIdx.z] : blockIdx.z) * sv;
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if(r >= N) return;
   cplx s = mk(0.0);
@@ -1680,18 +1685,21 @@ __global__ void gemv_reduce_kernel(const cplx *part, int ns, int N, cplx *y, siz
 // y = A x for nb systems (matrices sA, vectors sv apart); part holds
 // kGemvSplit N partial vectors per system
 void gemv64(const cplx *A, size_t lda, int N, const cplx *x, cplx *y, cplx *part, hipStream_t st, int nb, size_t sA,
-            size_t sv) {
+            size_t sv, const int *sys, int nsys) {
+  const int nz = sys ? nsys : nb;
+  if(nz < 1) return;
   int chunk = (N + kGemvSplit - 1) / kGemvSplit;
-  hipLaunchKernelGGL(gemv_part_kernel<cplx>, dim3((N + 255) / 256, kGemvSplit, nb), dim3(256), 0, st, A, lda, N, x,
-                     part, chunk, sA, sv);
-  hipLaunchKernelGGL(gemv_reduce_kernel, dim3((N + 255) / 256, 1, nb), dim3(256), 0, st, part, kGemvSplit, N, y, sv);
+  hipLaunchKernelGGL(gemv_part_kernel<cplx>, dim3((N + 255) / 256, kGemvSplit, nz), dim3(256), 0, st, A, lda, N, x,
+                     part, chunk, sA, sv, sys);
+  hipLaunchKernelGGL(gemv_reduce_kernel, dim3((N + 255) / 256, 1, nz), dim3(256), 0, st, part, kGemvSplit, N, y, sv,
+                     sys);
 }
 void gemv32(const float2 *A, size_t lda, int N, const cplx *x, cplx *y, cplx *part, hipStream_t st) {
   int chunk = (N + kGemvSplit - 1) / kGemvSplit;
   hipLaunchKernelGGL(gemv_part_kernel<float2>, dim3((N + 255) / 256, kGemvSplit, 1), dim3(256), 0, st, A, lda, N, x,
-                     part, chunk, (size_t)0, (size_t)0);
+                     part, chunk, (size_t)0, (size_t)0, (const int *)nullptr);
   hipLaunchKernelGGL(gemv_reduce_kernel, dim3((N + 255) / 256, 1, 1), dim3(256), 0, st, part, kGemvSplit, N, y,
-                     (size_t)0);
+                     (size_t)0, (const int *)nullptr);
 }
 
 }  // namespace omx
