@@ -68,6 +68,28 @@ def roofline(stats, kind, peak):
             "flop_per_launch": stats["lu_gemm_flop"] / n}
 
 
+def stage_rooflines(c, st, mixed_copy):
+    """The other stages of a step against their bounds, from the profiled
+    step's HIP-event times.  Assembly (objects, incident field, Q and the
+    pair kernel for S and V): algorithmic bytes = the matrices it stores,
+    16 N^2 per FP64 matrix plus 8 N^2 per complex-FP32 copy on the mixed path
+    (the pair kernel is bound by instruction issue, not by these stores: see
+    the counter profile).  Refinement matvecs: 16 Np^2 bytes per product."""
+    mats = 2 if c.sh else 1
+    store = mats * (16.0 + (8.0 if mixed_copy else 0.0)) * c.N * c.N
+    t = st["t_assemble_ms"] * 1e-3
+    out = {}
+    if t > 0:
+        gbs = store / t / 1e9
+        out["assembly"] = {"bound": "hbm", "kernel": "pair_kernel (S and V) with objects, incident and origin kernels",
+                           "achieved": gbs, "peak": HBM_GBS, "unit": "GB/s", "frac": gbs / HBM_GBS,
+                           "bytes": store, "ms": st["t_assemble_ms"]}
+    mv = matvec_roofline(st, "gemv_part_kernel (FP64 matvec of the refinement) + gemv_reduce_kernel")
+    if mv is not None:
+        out["refinement_matvec"] = mv
+    return out
+
+
 def pct(v, q):
     return float(np.percentile(np.asarray(v), q)) if v else float("nan")
 
@@ -148,6 +170,12 @@ def matvec_roofline(st, kernel="gemv_part_kernel (batched FP64 matvec of the ref
             "bytes_per_launch": st["matvec_bytes"] / st["matvec_calls"]}
 
 
+def tuned(Np, nb, fp64):
+    """The sweep autotuner's choice for a shape (see omx_set_autotune)."""
+    from optimet_amd import solver
+    return solver.tune_query(Np, nb, fp64)
+
+
 def sweep_main(args):
     """Config 3: one 16-sphere Drude cluster, nmax 10 (N = 3840), 256
     wavelengths in [400, 900] nm, omega and 2 omega, batched."""
@@ -171,6 +199,8 @@ def sweep_main(args):
         "p50_ms_per_sweep": 1e3 * pct(mixed["lat"], 50), "p99_ms_per_sweep": 1e3 * pct(mixed["lat"], 99),
         "max_iterations": [ms["iterations"], ms["iterations_sh"]], "max_residual": [ms["residual"], ms["residual_sh"]],
         "roofline": matvec_roofline(mixed["prof"]),
+        "autotune": {"lu32ir": tuned(ms["Np"], ms["sweep_batch"], False),
+                     "lu64": tuned(ms["Np"], ms["sweep_batch"], True) if fp64 is not None else None},
     }
     if fp64 is not None:
         fs = fp64["stats"]
@@ -522,6 +552,7 @@ def main():
         "lu_factor_ms_total": ms["t_factor_total_ms"],
         "lu_panel_ms_total": ms["t_lu_panel_ms"],
         "roofline": roof,
+        "stage_rooflines": stage_rooflines(c, ms, mixed_copy=True),
     }
     if fp64 is not None:
         fs = fp64["stats"]

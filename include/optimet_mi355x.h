@@ -144,6 +144,24 @@ int omx_set_solver(omx_context *ctx, int kind, double tol, int maxit, int restar
  * automatic choice. */
 int omx_set_auto_budget(omx_context *ctx, int steps);
 
+/* Autotuning of the batched sweeps (default on; no reference counterpart): per
+ * shape (device, padded order Np, batch nb, FP64 or FP32 factors) the sweep
+ * times each candidate bulk-update product of FP32 factors (split-bf16, FP32
+ * MFMA) and each triangular-solve variant on the batch's own buffers before
+ * its first chunk, and keeps the fastest (a candidate replaces the default
+ * only when at least 5% faster).  Off: the products of omx_set_lu32_gemm and
+ * the default triangular solves. */
+int omx_set_autotune(omx_context *ctx, int on);
+
+/* The tuned choice of a shape: *split (1 split-bf16, 0 FP32 MFMA), *trsv (the
+ * triangular-solve variant, 0 = default), times[5] (ms: the two products'
+ * probes, FP32 factors only, then the three triangular-solve variants).
+ * OMX_ERR_STATE when the shape has not been tuned in this process. */
+int omx_tune_query(int device, int Np, int nb, int fp64, int *split, int *trsv, double *times);
+
+/* Forget every tuned choice of this process. */
+int omx_tune_reset(void);
+
 /* Pivot search of the LU solvers (default OMX_PIVOT_BLOCK).  The reference
  * solves with Eigen's ColPivHouseholderQR (PreconditionedMatrixSolver.h:58,
  * 75) or ScaLAPACK pzgesv (scalapack/LinearSystemSolver.hpp:84-91); both

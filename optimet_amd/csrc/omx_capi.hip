@@ -10,7 +10,9 @@
 #include <functional>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -105,6 +107,7 @@ struct omx_context {
   int kind = OMX_SOLVER_LU64;
   int kind_req = OMX_SOLVER_LU64;  // as set; OMX_SOLVER_AUTO resolves into kind at each update
   int auto_budget = 0;             // outer Krylov steps of OMX_SOLVER_AUTO (0: the cost model)
+  bool autotune = true;            // batched sweeps pick their bulk product and triangular solves by timing
   int pivot = OMX_PIVOT_BLOCK;
   // fraction of S's columns factored before V's factorisation starts
   // (OMX_SH_OVERLAP overrides; 0 starts both together)
@@ -1211,6 +1214,109 @@ omx::LuWork sweep_lu(omx_context *c, int nb, bool second) {
   return lw;
 }
 
+// ------------------------------------------------------------------ autotuner
+// Per shape (device, padded order, batch, factor type) the batched sweeps
+// pick the bulk-update product of FP32 factors (split-bf16 or FP32 MFMA) and
+// the triangular-solve variant by timing every candidate on the batch's own
+// buffers before its first chunk (zeroed matrices, identity permutations:
+// the kernels have no value-dependent branches, so the times are those of the
+// real work), once per process.  A candidate replaces the default only when
+// it is at least 5% faster, so tuning again returns the same choice unless
+// two candidates sit at that margin.  The choices are kept per process and
+// can be read back (omx_tune_query).
+struct TuneChoice {
+  bool split = true;
+  int trsv = 0;
+  double t_prod[2] = {0, 0};                   // ms: split-bf16, FP32 MFMA (FP32 factors only)
+  double t_trsv[omx::kTrsvVariants] = {0, 0, 0};  // ms per batched LU solve
+};
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, TuneChoice> g_tune;  // (device, Np, nb, fp64)
+constexpr double kTuneMargin = 0.95;
+
+double time_median(hipStream_t st, const std::function<void()> &f, int reps = 5) {
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  std::vector<float> ts;
+  try {
+    f();  // warm-up
+    for(int r = 0; r < reps; ++r) {
+      HIPCHK(hipEventRecord(a, st));
+      f();
+      HIPCHK(hipEventRecord(b, st));
+      HIPCHK(hipEventSynchronize(b));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, a, b));
+      ts.push_back(ms);
+    }
+  } catch(...) {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    throw;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  std::sort(ts.begin(), ts.end());
+  return ts[ts.size() / 2];
+}
+
+omx::LuWork sweep_lu(omx_context *c, int nb, bool second);
+
+// The tuned choice for this batch (tuning it first if needed); must run
+// before the chunk clears its solve flags.
+TuneChoice sweep_tune(omx_context *c, int nb, bool f64) {
+  const auto key = std::make_tuple(c->device, c->Np, nb, f64 ? 1 : 0);
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(key);
+    if(it != g_tune.end()) return it->second;
+  }
+  auto &w = c->sw;
+  const int Np = c->Np;
+  const size_t n2 = (size_t)Np * Np, lda = (size_t)Np;
+  hipStream_t st = c->st;
+  TuneChoice ch;
+  ch.split = c->lu32_gemm == OMX_GEMM_BF16X3;
+  void *F = f64 ? w.F64.p : w.S32.p;
+  const size_t fbytes = (f64 ? sizeof(double2) : sizeof(float2)) * n2 * nb;
+  HIPCHK(hipMemsetAsync(F, 0, fbytes, st));
+  if(!f64) {
+    float2 *A = static_cast<float2 *>(F);
+    ch.t_prod[0] = time_median(st, [&] { omx::bulk_update_probe<float2>(A, Np, lda, nb, n2, true, st); });
+    ch.t_prod[1] = time_median(st, [&] { omx::bulk_update_probe<float2>(A, Np, lda, nb, n2, false, st); });
+    const int dflt = ch.split ? 0 : 1, other = 1 - dflt;
+    if(ch.t_prod[other] < kTuneMargin * ch.t_prod[dflt]) ch.split = other == 0;
+    HIPCHK(hipMemsetAsync(F, 0, fbytes, st));
+  }
+  // triangular solves on zeroed factors and inverses, identity permutations
+  std::vector<int> ident((size_t)Np * nb);
+  for(int z = 0; z < nb; ++z)
+    for(int i = 0; i < Np; ++i) ident[(size_t)z * Np + i] = i;
+  HIPCHK(hipMemcpyAsync(w.perm.p, ident.data(), sizeof(int) * ident.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(w.dinv.p, 0, w.dinv.bytes, st));
+  HIPCHK(hipMemsetAsync(w.flags.p, 0, w.flags.bytes, st));
+  HIPCHK(hipMemsetAsync(w.r.p, 0, w.r.bytes, st));
+  omx::LuWork lw = sweep_lu(c, nb, false);
+  for(int v = 0; v < omx::kTrsvVariants; ++v) {
+    lw.trsv = v;
+    ch.t_trsv[v] = time_median(st, [&] {
+      if(f64)
+        omx::lu_solve<double2>(static_cast<const double2 *>(F), Np, lda, lw, w.r.as<cplx>(), w.w2.as<cplx>(), st);
+      else
+        omx::lu_solve<float2>(static_cast<const float2 *>(F), Np, lda, lw, w.r.as<cplx>(), w.w2.as<cplx>(), st);
+    });
+  }
+  int best = 0;
+  for(int v = 1; v < omx::kTrsvVariants; ++v)
+    if(ch.t_trsv[v] < ch.t_trsv[best]) best = v;
+  ch.trsv = ch.t_trsv[best] < kTuneMargin * ch.t_trsv[0] ? best : 0;
+  HIPCHK(hipStreamSynchronize(st));  // ident ends here
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune[key] = ch;
+  return ch;
+}
+
 // One batch of a wavelength sweep: nb systems (wavelengths lam[z], absolute
 // permittivities eps[z * P + i], eps_sh[z * P + i]) assembled, factorised and
 // refined side by side.  Systems whose factorisation hit a zero pivot or whose
@@ -1221,6 +1327,7 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
                  std::vector<int> &iters, std::vector<double> &resid) {
   const bool f64 = c->kind == OMX_SOLVER_LU64, sh = c->sh;
   sweep_alloc(c, nb, f64, sh);
+  const TuneChoice tune = c->autotune ? sweep_tune(c, nb, f64) : TuneChoice{c->lu32_gemm == OMX_GEMM_BF16X3, 0};
   auto &w = c->sw;
   const int P = c->P, nmax = c->nmax, N = c->N, Np = c->Np;
   const size_t n2 = (size_t)Np * Np, lda = (size_t)Np;
@@ -1322,9 +1429,13 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
   // on the second stream pair right after the assembly and runs beside S's
   // factorisation and refinement; V's refinement waits for it
   omx::LuWork lwS = sweep_lu(c, nb, false), lwV{};
+  lwS.split = tune.split;
+  lwS.trsv = tune.trsv;
   void *FV = nullptr;
   if(sh) {
     lwV = sweep_lu(c, nb, true);
+    lwV.split = tune.split;
+    lwV.trsv = tune.trsv;
     HIPCHK(hipEventRecord(c->xev[0], st));
     HIPCHK(hipStreamWaitEvent(c->st2, c->xev[0], 0));
     FV = factor(true, lwV, c->st2);
@@ -1655,6 +1766,35 @@ int omx_set_auto_budget(omx_context *c, int steps) {
     if(!c || steps < 0) throw OmxError(OMX_ERR_ARG, "invalid budget");
     c->auto_budget = steps;
   });
+}
+
+int omx_set_autotune(omx_context *c, int on) {
+  return guard([&] {
+    if(!c) throw OmxError(OMX_ERR_ARG, "null context");
+    c->autotune = on != 0;
+  });
+}
+
+int omx_tune_query(int device, int Np, int nb, int fp64, int *split, int *trsv, double *times) {
+  return guard([&] {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(std::make_tuple(device, Np, nb, fp64 ? 1 : 0));
+    if(it == g_tune.end()) throw OmxError(OMX_ERR_STATE, "shape not tuned");
+    const TuneChoice &ch = it->second;
+    if(split) *split = ch.split ? 1 : 0;
+    if(trsv) *trsv = ch.trsv;
+    if(times) {
+      times[0] = ch.t_prod[0];
+      times[1] = ch.t_prod[1];
+      for(int v = 0; v < omx::kTrsvVariants; ++v) times[2 + v] = ch.t_trsv[v];
+    }
+  });
+}
+
+int omx_tune_reset(void) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune.clear();
+  return OMX_OK;
 }
 
 int omx_set_pivoting(omx_context *c, int mode) {

@@ -183,6 +183,7 @@ struct LuWork {
   BatchStrides bs;      // their strides; the pointers above are system 0's
   const int *sys = nullptr;  // lu_solve on a subset: its nsys systems' indices (null: all nb)
   int nsys = 0;
+  int trsv = 0;              // triangular-solve variant (kTrsvVariants; 0: the default)
   bool split = true;             // FP32 factors: bulk updates on the split-bf16 MFMA
   hipEvent_t ev_mark = nullptr;  // when set: recorded on the bulk stream once the
   int mark_col = 0;              // factorisation has passed column mark_col, and
@@ -210,6 +211,15 @@ struct LuProf {
 // Factor the Np x Np complex matrix in place (P A = L U); T = double2 or float2.
 template <typename T>
 int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof = nullptr);
+// Triangular-solve variants: super-block size (64-row blocks per workgroup)
+// and when the near-phase tiles are loaded (0: as used, 1: at kernel start,
+// 2: after the far phase).  Variant 0 is the default of each type.
+constexpr int kTrsvVariants = 3;
+// One bulk update of the shape the factorisation's first panel pair makes
+// (M = N = Np - 128, K = 128) on nb matrices sA apart: a timing probe for the
+// autotuner (A's contents are overwritten).
+template <typename T>
+void bulk_update_probe(T *A, int Np, size_t lda, int nb, size_t sA, bool split, hipStream_t st);
 // Solve with the factors: x = A^{-1} b, vectors always complex FP64.
 template <typename T>
 void lu_solve(const T *A, int Np, size_t lda, LuWork &w, const cplx *b, cplx *x, hipStream_t st);

@@ -1605,25 +1605,25 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
 }
 
 // Super-block size and near-phase prefetch of the triangular solves (the
-// complex-FP64 tiles of a prefetch would not fit in the register file).
+// complex-FP64 tiles of a prefetch would not fit in the register file):
+// variant 0 is the default, 1 one block per workgroup (every link crosses
+// workgroups, least work per link), 2 four blocks per workgroup with the
+// near-phase tiles loaded as used (fewer cross-workgroup links).
 template <typename T> struct TrsvCfg;
 template <> struct TrsvCfg<float2> {
-  static constexpr int SB = 2;
-  static constexpr int PRE = 1;
+  static constexpr int SB[kTrsvVariants] = {2, 1, 4};
+  static constexpr int PRE[kTrsvVariants] = {1, 1, 0};
 };
 template <> struct TrsvCfg<double2> {
-  static constexpr int SB = 2;
-  static constexpr int PRE = 2;
+  static constexpr int SB[kTrsvVariants] = {2, 1, 4};
+  static constexpr int PRE[kTrsvVariants] = {2, 2, 0};
 };
 
-template <typename T>
-void lu_solve(const T *A, int Np, size_t lda, LuWork &w, const cplx *b, cplx *x, hipStream_t st) {
+template <typename T, int V>
+static void trsv_pair(const T *A, int Np, size_t lda, LuWork &w, cplx *x, int nz, hipStream_t st) {
+  constexpr int SB = TrsvCfg<T>::SB[V];
+  constexpr int PRE = TrsvCfg<T>::PRE[V];
   const T *dinv = reinterpret_cast<const T *>(w.dinv);
-  constexpr int SB = TrsvCfg<T>::SB;
-  constexpr int PRE = TrsvCfg<T>::PRE;
-  const int nz = w.sys ? w.nsys : w.nb;
-  if(nz < 1) return;
-  hipLaunchKernelGGL(gather_kernel, dim3((Np + 255) / 256, 1, nz), dim3(256), 0, st, b, w.perm, x, Np, w.bs, w.sys);
   const int nblk = Np / kNB, nsb = (nblk + SB - 1) / SB;
   int e1 = ++w.epoch;
   hipLaunchKernelGGL((trsv_kernel<T, true, SB, PRE>), dim3(nsb, 1, nz), dim3(256), 0, st, A, lda, Np, dinv, x,
@@ -1632,6 +1632,26 @@ void lu_solve(const T *A, int Np, size_t lda, LuWork &w, const cplx *b, cplx *x,
   hipLaunchKernelGGL((trsv_kernel<T, false, SB, PRE>), dim3(nsb, 1, nz), dim3(256), 0, st, A, lda, Np, dinv, x,
                      w.flags, e2, w.status, w.bs, w.sys);
 }
+
+template <typename T>
+void lu_solve(const T *A, int Np, size_t lda, LuWork &w, const cplx *b, cplx *x, hipStream_t st) {
+  const int nz = w.sys ? w.nsys : w.nb;
+  if(nz < 1) return;
+  hipLaunchKernelGGL(gather_kernel, dim3((Np + 255) / 256, 1, nz), dim3(256), 0, st, b, w.perm, x, Np, w.bs, w.sys);
+  switch(w.trsv) {
+  case 1: trsv_pair<T, 1>(A, Np, lda, w, x, nz, st); break;
+  case 2: trsv_pair<T, 2>(A, Np, lda, w, x, nz, st); break;
+  default: trsv_pair<T, 0>(A, Np, lda, w, x, nz, st); break;
+  }
+}
+
+template <typename T>
+void bulk_update_probe(T *A, int Np, size_t lda, int nb, size_t sA, bool split, hipStream_t st) {
+  if(Np <= 2 * kNB) return;
+  bulk_update<T>(A, lda, 2 * kNB, 2 * kNB, 0, Np - 2 * kNB, Np - 2 * kNB, st, nb, sA, split);
+}
+template void bulk_update_probe<double2>(double2 *, int, size_t, int, size_t, bool, hipStream_t);
+template void bulk_update_probe<float2>(float2 *, int, size_t, int, size_t, bool, hipStream_t);
 
 template int lu_factor<double2>(double2 *, int, size_t, LuWork &, hipStream_t, LuProf *);
 template int lu_factor<float2>(float2 *, int, size_t, LuWork &, hipStream_t, LuProf *);

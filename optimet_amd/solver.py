@@ -33,7 +33,8 @@ EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "o
            "omx_cross_sections", "omx_scattering_size", "omx_get_stats", "omx_read_matrix", "omx_read_vector",
            "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_set_lu32_gemm", "omx_solve_sweep", "omx_mem_info", "omx_trim",
            "omx_dense_solve_noalloc", "omx_read_matrix_block", "omx_absorption_cross_section_sh",
-           "omx_fields", "omx_aca_info", "omx_aca_block", "omx_set_auto_budget"]
+           "omx_fields", "omx_aca_info", "omx_aca_block", "omx_set_auto_budget", "omx_set_autotune",
+           "omx_tune_query", "omx_tune_reset"]
 
 
 class OmxError(RuntimeError):
@@ -100,6 +101,9 @@ def load_library(path=LIB_PATH):
     L.omx_aca_info.argtypes = [P, I, ctypes.POINTER(ctypes.c_int), D]
     L.omx_aca_block.argtypes = [P, I, I, I, ctypes.POINTER(ctypes.c_int), D, D]
     L.omx_set_auto_budget.argtypes = [P, I]
+    L.omx_set_autotune.argtypes = [P, I]
+    L.omx_tune_query.argtypes = [I, I, I, I, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), D]
+    L.omx_tune_reset.argtypes = []
     _lib = L
     return L
 
@@ -154,6 +158,10 @@ class Context:
     def trim(self):
         """Release the matrix-sized device buffers."""
         _check(self.L.omx_trim(self.h))
+
+    def set_autotune(self, on):
+        """Batched sweeps tune their products and triangular solves per shape (default on)."""
+        _check(self.L.omx_set_autotune(self.h, 1 if on else 0))
 
     def set_auto_budget(self, steps):
         """Outer Krylov steps the "auto" kind tries before the LU (0: the cost model)."""
@@ -364,3 +372,22 @@ class Solver:
 
     def scattering_size(self):
         return self.ctx.L.omx_scattering_size(self.ctx.h)
+
+
+def tune_query(Np, nb, fp64=False, device=0):
+    """The sweep autotuner's choice for a shape, or None when not tuned:
+    dict split (bool), trsv (variant), times_ms (product probes split-bf16
+    and FP32 MFMA, then the triangular-solve variants)."""
+    L = load_library()
+    sp, tv = ctypes.c_int(0), ctypes.c_int(0)
+    t = np.zeros(5)
+    rc = L.omx_tune_query(device, Np, nb, 1 if fp64 else 0, ctypes.byref(sp), ctypes.byref(tv), _d(t))
+    if rc == -5:
+        return None
+    _check(rc)
+    return {"split": bool(sp.value), "trsv": tv.value, "times_ms": t.tolist()}
+
+
+def tune_reset():
+    """Forget the tuned choices of this process."""
+    load_library().omx_tune_reset()
