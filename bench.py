@@ -553,6 +553,8 @@ def main():
         "lu_panel_ms_total": ms["t_lu_panel_ms"],
         "roofline": roof,
         "stage_rooflines": stage_rooflines(c, ms, mixed_copy=True),
+        "autotune": {"lu32ir": tuned(ms["Np"], 1, False),
+                     "lu64": tuned(ms["Np"], 1, True) if fp64 is not None else None},
     }
     if fp64 is not None:
         fs = fp64["stats"]

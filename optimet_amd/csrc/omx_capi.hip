@@ -107,7 +107,8 @@ struct omx_context {
   int kind = OMX_SOLVER_LU64;
   int kind_req = OMX_SOLVER_LU64;  // as set; OMX_SOLVER_AUTO resolves into kind at each update
   int auto_budget = 0;             // outer Krylov steps of OMX_SOLVER_AUTO (0: the cost model)
-  bool autotune = true;            // batched sweeps pick their bulk product and triangular solves by timing
+  bool autotune = true;            // sweeps and solves pick their bulk product and triangular solves by timing
+  bool tuned = false, tuned_split = true;  // the single-system choice of the last update
   int pivot = OMX_PIVOT_BLOCK;
   // fraction of S's columns factored before V's factorisation starts
   // (OMX_SH_OVERLAP overrides; 0 starts both together)
@@ -750,7 +751,7 @@ void factor_enqueue(omx_context *c, bool shf, int pv) {
   hipStream_t st = sys_stream(c, shf);
   HIPCHK(hipMemsetAsync((shf ? c->d_status2 : c->d_status).p, 0, 16, st));
   lu.pivot = pv;
-  lu.split = c->lu32_gemm == OMX_GEMM_BF16X3;
+  lu.split = c->tuned ? c->tuned_split : c->lu32_gemm == OMX_GEMM_BF16X3;
   omx::LuProf *prof = prof_begin(c, shf);
   int rc;
   if(c->kind == OMX_SOLVER_LU64) {
@@ -1317,6 +1318,59 @@ TuneChoice sweep_tune(omx_context *c, int nb, bool f64) {
   return ch;
 }
 
+// The single-system LU solves (omx_update / omx_solve) tune the same way,
+// with batch 1 and the context's own buffers (before the assembly fills
+// them); the timing launches advance the factorisation's solve epochs.
+TuneChoice single_tune(omx_context *c, bool f64) {
+  const auto key = std::make_tuple(c->device, c->Np, 1, f64 ? 1 : 0);
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(key);
+    if(it != g_tune.end()) return it->second;
+  }
+  const int Np = c->Np;
+  const size_t n2 = (size_t)Np * Np, lda = (size_t)Np;
+  hipStream_t st = c->st;
+  TuneChoice ch;
+  ch.split = c->lu32_gemm == OMX_GEMM_BF16X3;
+  void *F = f64 ? c->d_F64.p : c->d_S32.p;
+  const size_t fbytes = (f64 ? sizeof(double2) : sizeof(float2)) * n2;
+  HIPCHK(hipMemsetAsync(F, 0, fbytes, st));
+  if(!f64) {
+    float2 *A = static_cast<float2 *>(F);
+    ch.t_prod[0] = time_median(st, [&] { omx::bulk_update_probe<float2>(A, Np, lda, 1, 0, true, st); });
+    ch.t_prod[1] = time_median(st, [&] { omx::bulk_update_probe<float2>(A, Np, lda, 1, 0, false, st); });
+    const int dflt = ch.split ? 0 : 1, other = 1 - dflt;
+    if(ch.t_prod[other] < kTuneMargin * ch.t_prod[dflt]) ch.split = other == 0;
+    HIPCHK(hipMemsetAsync(F, 0, fbytes, st));
+  }
+  std::vector<int> ident(Np);
+  for(int i = 0; i < Np; ++i) ident[i] = i;
+  HIPCHK(hipMemcpyAsync(c->d_perm.p, ident.data(), sizeof(int) * Np, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(c->d_dinv.p, 0, c->d_dinv.bytes, st));
+  HIPCHK(hipMemsetAsync(c->d_r.p, 0, c->d_r.bytes, st));
+  omx::LuWork &lw = c->lu;
+  for(int v = 0; v < omx::kTrsvVariants; ++v) {
+    lw.trsv = v;
+    ch.t_trsv[v] = time_median(st, [&] {
+      if(f64)
+        omx::lu_solve<double2>(static_cast<const double2 *>(F), Np, lda, lw, c->d_r.as<cplx>(), c->d_w2.as<cplx>(),
+                               st);
+      else
+        omx::lu_solve<float2>(static_cast<const float2 *>(F), Np, lda, lw, c->d_r.as<cplx>(), c->d_w2.as<cplx>(), st);
+    });
+  }
+  lw.trsv = 0;
+  int best = 0;
+  for(int v = 1; v < omx::kTrsvVariants; ++v)
+    if(ch.t_trsv[v] < ch.t_trsv[best]) best = v;
+  ch.trsv = ch.t_trsv[best] < kTuneMargin * ch.t_trsv[0] ? best : 0;
+  HIPCHK(hipStreamSynchronize(st));  // ident ends here
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune[key] = ch;
+  return ch;
+}
+
 // One batch of a wavelength sweep: nb systems (wavelengths lam[z], absolute
 // permittivities eps[z * P + i], eps_sh[z * P + i]) assembled, factorised and
 // refined side by side.  Systems whose factorisation hit a zero pivot or whose
@@ -1852,6 +1906,14 @@ int omx_update(omx_context *c) {
       c->kind = c->N >= kAutoKrylovN ? OMX_SOLVER_GMRES32_IR : OMX_SOLVER_LU32_IR;
     alloc_all(c);
     alloc_matrices(c);
+    c->tuned = false;
+    if(c->autotune && is_lu(c->kind)) {
+      const TuneChoice t = single_tune(c, c->kind == OMX_SOLVER_LU64);
+      c->lu.trsv = t.trsv;
+      if(c->sh) c->lu_sh.trsv = t.trsv;
+      c->tuned = true;
+      c->tuned_split = t.split;
+    }
     const int P = c->P, nmax = c->nmax;
     HIPCHK(hipEventRecord(c->ev[0], c->st));
     HIPCHK(hipMemcpyAsync(c->d_xyz.p, c->xyz.data(), sizeof(double) * 3 * P, hipMemcpyHostToDevice, c->st));
