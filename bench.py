@@ -94,7 +94,7 @@ def pct(v, q):
     return float(np.percentile(np.asarray(v), q)) if v else float("nan")
 
 
-def run_path(cluster, kind, steps, warmup, profile, pivoting="block"):
+def run_path(cluster, kind, steps, warmup, profile, pivoting="block", graphs=False):
     """Times `steps` unprofiled steps (the value and latencies), then, with
     `profile`, runs one more step with per-launch HIP events on (which
     serialises the omega and 2 omega factorisations) for the stage and GEMM
@@ -103,6 +103,7 @@ def run_path(cluster, kind, steps, warmup, profile, pivoting="block"):
 
     ctx = solver.Context()
     s = solver.Solver(cluster, kind=kind, context=ctx, pivoting=pivoting)
+    ctx.set_graphs(graphs)
     for _ in range(warmup):
         s.update()
         s.solve()
@@ -129,13 +130,14 @@ def run_path(cluster, kind, steps, warmup, profile, pivoting="block"):
     return {"lat": lat, "total": total, "out": out, "cs": cs, "stats": st}
 
 
-def run_sweep(cluster, lams, kind, steps, warmup, batch):
+def run_sweep(cluster, lams, kind, steps, warmup, batch, graphs=False):
     """Batched wavelength sweeps (omx_solve_sweep): every step solves all
     wavelengths at omega and 2 omega."""
     from optimet_amd import solver
 
     ctx = solver.Context()
     s = solver.Solver(cluster, kind=kind, context=ctx)
+    ctx.set_graphs(graphs)
     for _ in range(warmup):
         s.sweep(lams, batch=batch, outputs=())
     lat = []
@@ -183,8 +185,9 @@ def sweep_main(args):
 
     c = config3_cluster()
     lams = config3_wavelengths(256)
-    mixed = run_sweep(c, lams, "lu32ir", args.steps, args.warmup, args.batch)
-    fp64 = None if args.no_fp64 else run_sweep(c, lams, "lu64", args.steps, args.warmup, args.batch)
+    graphs = args.graphs == "on"
+    mixed = run_sweep(c, lams, "lu32ir", args.steps, args.warmup, args.batch, graphs=graphs)
+    fp64 = None if args.no_fp64 else run_sweep(c, lams, "lu64", args.steps, args.warmup, args.batch, graphs=graphs)
     nl = lams.size
     value = args.steps * nl / mixed["total"]
     ms = mixed["stats"]
@@ -194,7 +197,8 @@ def sweep_main(args):
         "ms_per_step": 1e3 * mixed["total"] / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "c64 LU (FP32-accurate bf16x3 MFMA products) + c128 GMRES-IR", "data": "synthetic (seeded Drude cluster)",
         "config": {"workload": c.meta["name"], "P": c.nobj, "nmax": c.nmax, "N": c.N, "wavelengths": nl,
-                   "frequencies": "omega+2omega", "solver": "lu32ir", "batch": ms["sweep_batch"]},
+                   "frequencies": "omega+2omega", "solver": "lu32ir", "batch": ms["sweep_batch"],
+                   "graphs": args.graphs},
         "step": "one sweep of all wavelengths (a solve = one wavelength at omega and 2 omega)",
         "p50_ms_per_sweep": 1e3 * pct(mixed["lat"], 50), "p99_ms_per_sweep": 1e3 * pct(mixed["lat"], 99),
         "max_iterations": [ms["iterations"], ms["iterations_sh"]], "max_residual": [ms["residual"], ms["residual_sh"]],
@@ -483,6 +487,8 @@ def main():
                     help="config5: requests of the profiled replay the roofline comes from")
     ap.add_argument("--deadline", type=float, default=None, help="config5: admission closes after this many s")
     ap.add_argument("--batch", type=int, default=0, help="systems per batch of the config3 sweep (0: automatic)")
+    ap.add_argument("--graphs", choices=("on", "off"), default="off",
+                    help="config2/config3: factorisations captured in HIP graphs and replayed (omx_set_graphs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer (calibration of rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -515,11 +521,16 @@ def main():
     from optimet_amd.problem import config2
 
     c = config2()
-    mixed = run_path(c, "lu32ir", args.steps, args.warmup, True, pivoting=args.pivoting)
-    fp64 = None if args.no_fp64 else run_path(c, "lu64", args.steps, args.warmup, True)
+    graphs = args.graphs == "on"
+    # the value: the automatic solver choice (S by FP64 GMRES beside V's mixed
+    # LU, S's own mixed LU if GMRES falls short); beside it the mixed LU for
+    # both systems and the FP64 LU of the metric's comparison
+    best = run_path(c, "auto", args.steps, args.warmup, True, pivoting=args.pivoting, graphs=graphs)
+    mixed = run_path(c, "lu32ir", args.steps, args.warmup, True, pivoting=args.pivoting, graphs=graphs)
+    fp64 = None if args.no_fp64 else run_path(c, "lu64", args.steps, args.warmup, True, graphs=graphs)
 
-    ms = mixed["stats"]
-    value = args.steps / mixed["total"]
+    ms = best["stats"]
+    value = args.steps / best["total"]
     roof = roofline(ms, "lu32ir", SPLIT_TFLOPS)
     if roof is not None:
         roof["peak_basis"] = "bf16 dense 2500 TFLOP/s / 6 bf16 products per FP32-accurate real product"
@@ -536,16 +547,19 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * mixed["total"] / args.steps,
+        "ms_per_step": 1e3 * best["total"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 LU (FP32-accurate bf16x3 MFMA products) + c128 GMRES-IR",
+        "dtype": "c128 GMRES (omega) beside c64 LU (FP32-accurate bf16x3 MFMA products) + c128 GMRES-IR (2 omega)",
         "data": "synthetic (seeded Drude cluster)",
         "config": {"workload": c.meta["name"], "P": c.nobj, "nmax": c.nmax, "N": c.N,
-                   "wavelength_nm": c.wavelength * 1e9, "frequencies": "omega+2omega", "solver": "lu32ir"},
-        "p50_ms": 1e3 * pct(mixed["lat"], 50),
-        "p99_ms": 1e3 * pct(mixed["lat"], 99),
+                   "wavelength_nm": c.wavelength * 1e9, "frequencies": "omega+2omega", "solver": "auto",
+                   "graphs": args.graphs},
+        "solver_choice": {"omega": "FP64 GMRES" if ms["pivot_fallbacks"] == 0 else "mixed LU (GMRES fell short)",
+                          "2omega": "mixed LU + FP64 GMRES-IR"},
+        "p50_ms": 1e3 * pct(best["lat"], 50),
+        "p99_ms": 1e3 * pct(best["lat"], 99),
         "stages_ms": {k: ms[k] for k in ("t_update_ms", "t_assemble_ms", "t_solve_ms", "t_sh_source_ms")},
         "iterations": [ms["iterations"], ms["iterations_sh"]],
         "residual": [ms["residual"], ms["residual_sh"]],
@@ -553,6 +567,11 @@ def main():
         "lu_panel_ms_total": ms["t_lu_panel_ms"],
         "roofline": roof,
         "stage_rooflines": stage_rooflines(c, ms, mixed_copy=True),
+        "mixed_lu": {"solves_per_s": args.steps / mixed["total"], "p50_ms": 1e3 * pct(mixed["lat"], 50),
+                     "p99_ms": 1e3 * pct(mixed["lat"], 99),
+                     "iterations": [mixed["stats"]["iterations"], mixed["stats"]["iterations_sh"]],
+                     "roofline": roofline(mixed["stats"], "lu32ir", SPLIT_TFLOPS),
+                     "lu_factor_ms_total": mixed["stats"]["t_factor_total_ms"]},
         "autotune": {"lu32ir": tuned(ms["Np"], 1, False),
                      "lu64": tuned(ms["Np"], 1, True) if fp64 is not None else None},
     }
@@ -565,12 +584,17 @@ def main():
                            "iterations": [fs["iterations"], fs["iterations_sh"]],
                            "lu_factor_ms_total": fs["t_factor_total_ms"], "lu_panel_ms_total": fs["t_lu_panel_ms"]}
         line["speedup_vs_fp64"] = value / line["fp64_lu"]["solves_per_s"]
-        mo, fo = mixed["out"], fp64["out"]
-        line["rel_err_vs_fp64"] = {"x_sca": rel(mo["x_sca"], fo["x_sca"]), "x_int": rel(mo["x_int"], fo["x_int"]),
-                                   "x_sca_sh": rel(mo["x_sca_sh"], fo["x_sca_sh"]),
-                                   "cext": abs(mixed["cs"]["cext"] - fp64["cs"]["cext"]) / abs(fp64["cs"]["cext"]),
-                                   "csca_sh": abs(mixed["cs"]["csca_sh"] - fp64["cs"]["csca_sh"])
-                                   / abs(fp64["cs"]["csca_sh"])}
+        line["mixed_lu"]["speedup_vs_fp64"] = line["mixed_lu"]["solves_per_s"] / line["fp64_lu"]["solves_per_s"]
+        fo = fp64["out"]
+
+        def errs(run):
+            mo = run["out"]
+            return {"x_sca": rel(mo["x_sca"], fo["x_sca"]), "x_int": rel(mo["x_int"], fo["x_int"]),
+                    "x_sca_sh": rel(mo["x_sca_sh"], fo["x_sca_sh"]),
+                    "cext": abs(run["cs"]["cext"] - fp64["cs"]["cext"]) / abs(fp64["cs"]["cext"]),
+                    "csca_sh": abs(run["cs"]["csca_sh"] - fp64["cs"]["csca_sh"]) / abs(fp64["cs"]["csca_sh"])}
+        line["rel_err_vs_fp64"] = errs(best)
+        line["mixed_lu"]["rel_err_vs_fp64"] = errs(mixed)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(c)
     print(json.dumps(line), flush=True)

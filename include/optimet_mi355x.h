@@ -107,6 +107,8 @@ typedef struct {
   int matvec_calls;      /* FP64 matrix-vector products of the iterative solvers (since omx_set_profile) */
   double t_matvec_ms;    /* their HIP-event time (profiling on) */
   double matvec_bytes;   /* their algorithmic bytes, 16 Np^2 per system and product */
+  int graph_captures;    /* factorisation graphs captured (omx_set_graphs) since creation */
+  int graph_launches;    /* factorisations run as graph launches since creation */
 } omx_stats;
 
 const char *omx_last_error(void);
@@ -161,6 +163,14 @@ int omx_tune_query(int device, int Np, int nb, int fp64, int *split, int *trsv, 
 
 /* Forget every tuned choice of this process. */
 int omx_tune_reset(void);
+
+/* No reference counterpart.  On: the block-pivoting factorisations of
+ * omx_solve and of omx_solve_sweep's chunks are captured in HIP graphs once
+ * per state (order, batch, solver kind, product, buffers) and replayed;
+ * S's graph is cut at V's start mark so that V's factorisation still overlaps
+ * S's.  The kernels and their order are those of the eager path, so results
+ * are identical.  Off (default): eager launches, and the graphs are released. */
+int omx_set_graphs(omx_context *ctx, int on);
 
 /* Pivot search of the LU solvers (default OMX_PIVOT_BLOCK).  The reference
  * solves with Eigen's ColPivHouseholderQR (PreconditionedMatrixSolver.h:58,

@@ -82,6 +82,14 @@ constexpr double kInnerTol = 1e-6;
 // does not converge; smaller systems go to the LU directly.
 constexpr int kAutoKrylovN = 16384;
 inline int auto_krylov_steps(int N) { return std::max(4, (int)(0.0143 * N / (kInnerSteps + 1))); }
+// Below kAutoKrylovN, with the second harmonic: V's mixed LU starts at once on
+// the second stream pair (V does not depend on the fundamental solution), and
+// S is first tried with FP64 GMRES beside it, so that S's solve is hidden
+// while it takes less than V's factorisation.  Its budget is the products
+// that cost what an LU would (8/3 N^3 flop at about 125 TFLOP/s against a
+// 16 N^2-byte FP64 product at about 6 TB/s: N / 125); S falls back to its own
+// mixed LU when GMRES has not converged by then.
+inline int auto_gmres64_steps(int N) { return std::max(20, N / 125); }
 
 const double kMu0 = 4.0 * omx::kPi * 1e-7;
 const double kEps0 = 1.0 / (kMu0 * omx::kC0 * omx::kC0);
@@ -107,6 +115,7 @@ struct omx_context {
   int kind = OMX_SOLVER_LU64;
   int kind_req = OMX_SOLVER_LU64;  // as set; OMX_SOLVER_AUTO resolves into kind at each update
   int auto_budget = 0;             // outer Krylov steps of OMX_SOLVER_AUTO (0: the cost model)
+  bool krylov_s = false;           // OMX_SOLVER_AUTO: S tried with FP64 GMRES beside V's factorisation
   bool autotune = true;            // sweeps and solves pick their bulk product and triangular solves by timing
   bool tuned = false, tuned_split = true;  // the single-system choice of the last update
   int pivot = OMX_PIVOT_BLOCK;
@@ -191,6 +200,23 @@ struct omx_context {
       h_pin_n = 0;
     }
   } ws_in;
+  // Captured factorisations (omx_set_graphs): executable graphs of S's
+  // factorisation (s1, then s2 after V's start mark) and V's, and the state
+  // they were captured for; one set for single solves, one for sweep chunks
+  struct FactorGraphs {
+    hipGraphExec_t s1 = nullptr, s2 = nullptr, v = nullptr;
+    std::vector<uintptr_t> key;
+    void reset() {
+      hipGraphExec_t *g[] = {&s1, &s2, &v};
+      for(hipGraphExec_t *x : g) {
+        if(*x) (void)hipGraphExecDestroy(*x);
+        *x = nullptr;
+      }
+      key.clear();
+    }
+  } fg, fg_sweep;
+  bool graphs = false;
+  int graph_captures = 0, graph_launches = 0;
   omx_stats stats{};
   void *scratch = nullptr;  // this device's address of the no-allocation solve's scratch array
   long long mv_calls = 0;  // FP64 matrix-vector products since the last omx_set_profile
@@ -749,7 +775,7 @@ void factor_enqueue(omx_context *c, bool shf, int pv) {
   const int Np = c->Np;
   const size_t lda = (size_t)Np, n2 = lda * Np;
   hipStream_t st = sys_stream(c, shf);
-  HIPCHK(hipMemsetAsync((shf ? c->d_status2 : c->d_status).p, 0, 16, st));
+  omx::zero_ints((shf ? c->d_status2 : c->d_status).as<int>(), 4, st);  // a kernel: this may be captured
   lu.pivot = pv;
   lu.split = c->tuned ? c->tuned_split : c->lu32_gemm == OMX_GEMM_BF16X3;
   omx::LuProf *prof = prof_begin(c, shf);
@@ -768,6 +794,146 @@ void factor_enqueue(omx_context *c, bool shf, int pv) {
   if(rc) throw OmxError(OMX_ERR_DEVICE, "lu_factor launch failed (" + std::to_string(rc) + ")");
   prof_end(c, shf);
   (shf ? c->factored_sh : c->factored) = true;
+}
+
+// ------------------------------------------------------------------ graphs
+// With omx_set_graphs on, the block-pivoting factorisations are captured once
+// per state (the key: order, kind, pivoting, product, whether the FP32 copy
+// must be refreshed first, and every buffer address the kernels take) and
+// replayed afterwards.  S's capture is cut at V's start mark into two graphs
+// and V's factorisation is captured on its own stream pair, so a replay keeps
+// the eager overlap: S's first part, then V on the second stream after it
+// beside S's rest.  Inside the captures every clear is a kernel.
+
+void end_capture_quietly(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if(hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(st, &g);
+    if(g) (void)hipGraphDestroy(g);
+  }
+  (void)hipGetLastError();
+}
+
+hipGraphExec_t instantiate(hipGraph_t g) {
+  if(!g) return nullptr;
+  size_t n = 0;
+  HIPCHK(hipGraphGetNodes(g, nullptr, &n));
+  hipGraphExec_t e = nullptr;
+  if(n) HIPCHK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+  return e;
+}
+
+std::vector<uintptr_t> single_key(omx_context *c, bool with_v, int mark) {
+  const bool split = c->tuned ? c->tuned_split : c->lu32_gemm == OMX_GEMM_BF16X3;
+  std::vector<uintptr_t> k{1,
+                           (uintptr_t)with_v,
+                           (uintptr_t)c->Np,
+                           (uintptr_t)c->kind,
+                           (uintptr_t)c->pivot,
+                           (uintptr_t)split,
+                           (uintptr_t)mark,
+                           (uintptr_t)c->dirty32,
+                           (uintptr_t)(with_v && c->dirty32_sh)};
+  const omx::LuWork *lw[] = {&c->lu, &c->lu_sh};
+  for(const omx::LuWork *w : lw) {
+    const void *p[] = {w->perm, w->mdst, w->msrc, w->mcnt, w->status, w->dinv, w->aux};
+    for(const void *x : p) k.push_back((uintptr_t)x);
+  }
+  const void *b[] = {c->d_S64.p, c->d_S32.p, c->d_V64.p, c->d_V32.p, c->d_F64.p, c->d_F64sh.p, c->d_status.p,
+                     c->d_status2.p};
+  for(const void *x : b) k.push_back((uintptr_t)x);
+  return k;
+}
+
+struct CaptureMark {
+  omx_context *c;
+  hipGraph_t g1 = nullptr, gv = nullptr;
+};
+
+// At V's start mark inside S's capture: close S's first graph, capture V's
+// factorisation on the second stream pair, and reopen S's capture (the
+// look-ahead's next waits are on events recorded in the new capture).
+void capture_at_mark(void *arg) {
+  auto *cm = static_cast<CaptureMark *>(arg);
+  omx_context *c = cm->c;
+  HIPCHK(hipStreamEndCapture(c->st, &cm->g1));
+  HIPCHK(hipStreamBeginCapture(c->st2, hipStreamCaptureModeThreadLocal));
+  factor_enqueue(c, true, c->pivot);
+  HIPCHK(hipStreamEndCapture(c->st2, &cm->gv));
+  HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  HIPCHK(hipEventRecord(c->lu.ev_next, c->st));
+  HIPCHK(hipEventRecord(c->lu.ev_bulk, c->st));
+}
+
+// Enqueues S's factorisation (and V's from S's mark column on, with_v) as
+// graph launches, capturing them first when the state has changed.
+void factor_graphed(omx_context *c, bool with_v, int mark) {
+  auto &g = c->fg;
+  std::vector<uintptr_t> key = single_key(c, with_v, mark);
+  if(g.key != key) {
+    g.reset();
+    CaptureMark cm{c};
+    hipGraph_t g2 = nullptr;
+    omx::LuWork &lu = c->lu;
+    auto unmark = [&] {
+      lu.ev_mark = nullptr;
+      lu.on_mark = nullptr;
+      lu.mark_arg = nullptr;
+    };
+    const bool d32 = c->dirty32, d32sh = c->dirty32_sh;
+    try {
+      HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+      if(with_v) {
+        lu.ev_mark = c->xev[0];
+        lu.mark_col = mark;
+        lu.on_mark = capture_at_mark;
+        lu.mark_arg = &cm;
+      }
+      factor_enqueue(c, false, c->pivot);
+      unmark();
+      HIPCHK(hipStreamEndCapture(c->st, with_v ? &g2 : &cm.g1));
+      if(with_v && !cm.gv) throw OmxError(OMX_ERR_DEVICE, "factorisation capture: V's start mark was not reached");
+      g.s1 = instantiate(cm.g1);
+      g.s2 = instantiate(g2);
+      g.v = instantiate(cm.gv);
+    } catch(...) {
+      unmark();
+      end_capture_quietly(c->st);
+      end_capture_quietly(c->st2);
+      hipGraph_t gs[] = {cm.g1, cm.gv, g2};
+      for(hipGraph_t x : gs)
+        if(x) (void)hipGraphDestroy(x);
+      g.reset();
+      c->dirty32 = d32;
+      c->dirty32_sh = d32sh;
+      c->factored = c->factored_sh = false;
+      throw;
+    }
+    hipGraph_t gs[] = {cm.g1, cm.gv, g2};
+    for(hipGraph_t x : gs)
+      if(x) (void)hipGraphDestroy(x);
+    g.key = key;
+    ++c->graph_captures;
+  }
+  // what factor_enqueue leaves behind
+  const bool split = c->tuned ? c->tuned_split : c->lu32_gemm == OMX_GEMM_BF16X3;
+  c->lu.pivot = c->pivot;
+  c->lu.split = split;
+  if(c->kind != OMX_SOLVER_LU64) c->dirty32 = true;
+  c->factored = true;
+  if(g.s1) HIPCHK(hipGraphLaunch(g.s1, c->st));
+  if(with_v) {
+    c->lu_sh.pivot = c->pivot;
+    c->lu_sh.split = split;
+    if(c->kind != OMX_SOLVER_LU64) c->dirty32_sh = true;
+    c->factored_sh = true;
+    HIPCHK(hipEventRecord(c->xev[0], c->st));
+    HIPCHK(hipStreamWaitEvent(c->st2, c->xev[0], 0));
+    if(g.v) HIPCHK(hipGraphLaunch(g.v, c->st2));
+  }
+  if(g.s2) HIPCHK(hipGraphLaunch(g.s2, c->st));
+  ++c->graph_launches;
 }
 
 // The target may sit below the FP64 residual floor of an ill-conditioned
@@ -846,6 +1012,16 @@ void solve_system(omx_context *c, bool shf, const cplx *rhs, cplx *x, int *iters
       c->kind = OMX_SOLVER_LU32_IR;  // the same S and S32: on to the mixed LU
       ++c->fallbacks;
     }
+  }
+  if(!shf && c->krylov_s && !fact) {
+    // FP64 GMRES on S while V factorises; S's own mixed LU if it falls short
+    omx::zero_ints(status.as<int>(), 4, st);
+    HIPCHK(hipMemsetAsync(x, 0, sizeof(cplx) * Np, st));
+    const int maxit = std::min(c->maxit, c->auto_budget > 0 ? c->auto_budget : auto_gmres64_steps(c->N));
+    *iters = gm.solve(rhs, x, c->tol, maxit, c->restart, res);
+    check_status(c, shf);
+    if(*res <= 1e3 * c->tol || backward_stable(c, gm, A64, rhs, x, *res, st)) return;
+    ++c->fallbacks;
   }
   const bool f64 = c->kind == OMX_SOLVER_LU64;
   cplx *F64 = (shf ? c->d_F64sh : c->d_F64).as<cplx>();
@@ -1485,17 +1661,63 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
   omx::LuWork lwS = sweep_lu(c, nb, false), lwV{};
   lwS.split = tune.split;
   lwS.trsv = tune.trsv;
-  void *FV = nullptr;
   if(sh) {
     lwV = sweep_lu(c, nb, true);
     lwV.split = tune.split;
     lwV.trsv = tune.trsv;
-    HIPCHK(hipEventRecord(c->xev[0], st));
-    HIPCHK(hipStreamWaitEvent(c->st2, c->xev[0], 0));
-    FV = factor(true, lwV, c->st2);
-    HIPCHK(hipEventRecord(c->xev[1], c->st2));
   }
-  void *FS = factor(false, lwS, st);
+  void *FS = f64 ? w.F64.p : w.S32.p, *FV = sh ? (f64 ? w.F64sh.p : w.V32.p) : nullptr;
+  if(c->graphs) {
+    // captured once per batch shape and buffers, then replayed chunk after chunk
+    auto &g = c->fg_sweep;
+    std::vector<uintptr_t> key{2, (uintptr_t)nb, (uintptr_t)Np, (uintptr_t)f64, (uintptr_t)sh, (uintptr_t)tune.split};
+    const void *p[] = {w.S64.p, w.S32.p, w.V64.p, w.V32.p, w.F64.p, w.F64sh.p, w.perm.p, w.perm2.p, w.map.p,
+                       w.map2.p, w.status.p, w.status2.p, w.dinv.p, w.dinv2.p};
+    for(const void *x : p) key.push_back((uintptr_t)x);
+    if(g.key != key) {
+      g.reset();
+      hipGraph_t gs = nullptr, gv = nullptr;
+      try {
+        if(sh) {
+          HIPCHK(hipStreamBeginCapture(c->st2, hipStreamCaptureModeThreadLocal));
+          factor(true, lwV, c->st2);
+          HIPCHK(hipStreamEndCapture(c->st2, &gv));
+        }
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        factor(false, lwS, st);
+        HIPCHK(hipStreamEndCapture(st, &gs));
+        g.s1 = instantiate(gs);
+        g.v = instantiate(gv);
+      } catch(...) {
+        end_capture_quietly(c->st2);
+        end_capture_quietly(st);
+        if(gs) (void)hipGraphDestroy(gs);
+        if(gv) (void)hipGraphDestroy(gv);
+        g.reset();
+        throw;
+      }
+      if(gs) (void)hipGraphDestroy(gs);
+      if(gv) (void)hipGraphDestroy(gv);
+      g.key = key;
+      ++c->graph_captures;
+    }
+    if(sh) {
+      HIPCHK(hipEventRecord(c->xev[0], st));
+      HIPCHK(hipStreamWaitEvent(c->st2, c->xev[0], 0));
+      if(g.v) HIPCHK(hipGraphLaunch(g.v, c->st2));
+      HIPCHK(hipEventRecord(c->xev[1], c->st2));
+    }
+    if(g.s1) HIPCHK(hipGraphLaunch(g.s1, st));
+    ++c->graph_launches;
+  } else {
+    if(sh) {
+      HIPCHK(hipEventRecord(c->xev[0], st));
+      HIPCHK(hipStreamWaitEvent(c->st2, c->xev[0], 0));
+      factor(true, lwV, c->st2);
+      HIPCHK(hipEventRecord(c->xev[1], c->st2));
+    }
+    factor(false, lwS, st);
+  }
   // refinement of every system of S (V); per-system convergence flags
   auto refine = [&](bool second, omx::LuWork &lw, void *F, const cplx *rhs, cplx *x) {
     status_scan(bad, second);
@@ -1700,6 +1922,8 @@ void omx_destroy(omx_context *c) {
   (void)hipSetDevice(c->device);
   if(c->st) (void)hipStreamSynchronize(c->st);
   if(c->st2) (void)hipStreamSynchronize(c->st2);
+  c->fg.reset();
+  c->fg_sweep.reset();
   c->d_status2.release(&c->total);
   c->d_map2.release(&c->total);
   DBuf *bufs[] = {&c->d_xyz, &c->d_radius, &c->d_eps, &c->d_mu, &c->d_eps_sh, &c->d_kppp, &c->d_kpar, &c->d_gam,
@@ -1822,6 +2046,20 @@ int omx_set_auto_budget(omx_context *c, int steps) {
   });
 }
 
+int omx_set_graphs(omx_context *c, int on) {
+  return guard([&] {
+    if(!c) throw OmxError(OMX_ERR_ARG, "null context");
+    HIPCHK(hipSetDevice(c->device));
+    c->graphs = on != 0;
+    if(!c->graphs) {
+      HIPCHK(hipStreamSynchronize(c->st));
+      HIPCHK(hipStreamSynchronize(c->st2));
+      c->fg.reset();
+      c->fg_sweep.reset();
+    }
+  });
+}
+
 int omx_set_autotune(omx_context *c, int on) {
   return guard([&] {
     if(!c) throw OmxError(OMX_ERR_ARG, "null context");
@@ -1904,6 +2142,7 @@ int omx_update(omx_context *c) {
     HIPCHK(hipSetDevice(c->device));
     if(c->kind_req == OMX_SOLVER_AUTO)
       c->kind = c->N >= kAutoKrylovN ? OMX_SOLVER_GMRES32_IR : OMX_SOLVER_LU32_IR;
+    c->krylov_s = c->kind_req == OMX_SOLVER_AUTO && c->kind == OMX_SOLVER_LU32_IR && c->sh;
     alloc_all(c);
     alloc_matrices(c);
     c->tuned = false;
@@ -1988,6 +2227,8 @@ int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, do
     // fundamental solve (not while profiling, which times each factorisation
     // alone)
     const bool lu_kind = is_lu(c->kind);
+    const bool graphed =
+        c->graphs && lu_kind && !c->profile && c->pivot == OMX_PIVOT_BLOCK && !c->factored && !c->krylov_s;
     if(c->sh && lu_kind && !c->factored_sh && !c->profile) {
       // V's factorisation starts once S's has passed a fraction of its
       // columns (by then S's trailing updates no longer fill the device),
@@ -1995,13 +2236,15 @@ int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, do
       // (V's work is enqueued right at the mark, in the middle of S's
       // enqueueing: a wait on the mark event enqueued only after all of S's
       // work was seen to hold V back until S's end)
-      const int mark = (int)(c->sh_overlap * c->Np);
+      const int mark = c->krylov_s ? 0 : (int)(c->sh_overlap * c->Np);
       auto start_v = [](void *arg) {
         omx_context *cc = static_cast<omx_context *>(arg);
         HIPCHK(hipStreamWaitEvent(cc->st2, cc->xev[0], 0));
         factor_enqueue(cc, true, cc->pivot);
       };
-      if(mark > 0 && !c->factored) {
+      if(mark > 0 && graphed) {
+        factor_graphed(c, true, mark);
+      } else if(mark > 0 && !c->factored) {
         c->lu.ev_mark = c->xev[0];
         c->lu.mark_col = mark;
         c->lu.on_mark = start_v;
@@ -2020,6 +2263,7 @@ int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, do
         start_v(c);
       }
     }
+    if(graphed && !c->factored) factor_graphed(c, false, 0);
     // X_sca = S^{-1} Q (PreconditionedMatrixSolver.h:58)
     solve_system(c, false, c->d_Q.as<cplx>(), c->d_x.as<cplx>(), &c->stats.iterations, &c->stats.residual);
     HIPCHK(hipEventRecord(c->ev[1], c->st));
@@ -2167,6 +2411,8 @@ int omx_get_stats(const omx_context *c, omx_stats *out) {
   out->matvec_calls = (int)c->mv_calls;
   out->t_matvec_ms = c->mv_ms;
   out->matvec_bytes = c->mv_bytes;
+  out->graph_captures = c->graph_captures;
+  out->graph_launches = c->graph_launches;
   return OMX_OK;
 }
 
@@ -2465,6 +2711,9 @@ int omx_trim(omx_context *c) {
     if(!c) throw OmxError(OMX_ERR_ARG, "null context");
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->st2));
+    c->fg.reset();
+    c->fg_sweep.reset();
     auto &w = c->sw;
     DBuf *big[] = {&c->d_S64, &c->d_S32, &c->d_V64, &c->d_V32, &c->d_F64, &c->d_F64sh, &w.S64, &w.S32, &w.V64,
                    &w.V32, &w.F64, &w.F64sh, &w.basis, &w.basisZ, &w.gpart, &c->d_basis, &c->d_basisZ,

@@ -110,6 +110,11 @@ __global__ void demote_k(float2 *y, const cplx *x, size_t n) {
 void demote(float2 *y, const cplx *x, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(demote_k, dim3(4096), dim3(256), 0, st, y, x, n);
 }
+__global__ void zero_ints_k(int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) p[i] = 0;
+}
+void zero_ints(int *p, int n, hipStream_t st) { hipLaunchKernelGGL(zero_ints_k, g1(n), dim3(256), 0, st, p, n); }
 void vcopy(cplx *y, const cplx *x, int n, hipStream_t st) { hipLaunchKernelGGL(vcopy_k, g1(n), dim3(256), 0, st, y, x, n); }
 void vaxpby(cplx *y, cplx a, const cplx *x, cplx b, int n, hipStream_t st) {
   hipLaunchKernelGGL(vaxpby_k, g1(n), dim3(256), 0, st, y, a, x, b, n);

@@ -235,6 +235,8 @@ constexpr int kGemvSplitMax = 32;
 // level-1 helpers (omx_iter.hip); reductions return through small device arrays
 void vcopy(cplx *y, const cplx *x, int n, hipStream_t st);
 void demote(float2 *y, const cplx *x, size_t n, hipStream_t st);                 // y = (c64) x
+// p[0..n) = 0 by a kernel (a memset node must not go into a captured graph)
+void zero_ints(int *p, int n, hipStream_t st);
 void vaxpby(cplx *y, cplx a, const cplx *x, cplx b, int n, hipStream_t st);   // y = a x + b y
 void vsub(cplx *z, const cplx *x, const cplx *y, int n, hipStream_t st);      // z = x - y
 void vmul_ov(cplx *y, const cplx *x, const cplx *ov, int kind, int P, int nmax, const cplx *sub, int n,

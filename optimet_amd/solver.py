@@ -34,7 +34,7 @@ EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "o
            "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_set_lu32_gemm", "omx_solve_sweep", "omx_mem_info", "omx_trim",
            "omx_dense_solve_noalloc", "omx_read_matrix_block", "omx_absorption_cross_section_sh",
            "omx_fields", "omx_aca_info", "omx_aca_block", "omx_set_auto_budget", "omx_set_autotune",
-           "omx_tune_query", "omx_tune_reset"]
+           "omx_tune_query", "omx_tune_reset", "omx_set_graphs"]
 
 
 class OmxError(RuntimeError):
@@ -52,7 +52,8 @@ class Stats(ctypes.Structure):
                 ("lu_gemm_flop", ctypes.c_double), ("lu_gemm_launches", ctypes.c_int),
                 ("t_lu_panel_ms", ctypes.c_double), ("t_factor_total_ms", ctypes.c_double),
                 ("pivot_fallbacks", ctypes.c_int), ("sweep_batch", ctypes.c_int), ("matvec_calls", ctypes.c_int),
-                ("t_matvec_ms", ctypes.c_double), ("matvec_bytes", ctypes.c_double)]
+                ("t_matvec_ms", ctypes.c_double), ("matvec_bytes", ctypes.c_double),
+                ("graph_captures", ctypes.c_int), ("graph_launches", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -104,6 +105,7 @@ def load_library(path=LIB_PATH):
     L.omx_set_autotune.argtypes = [P, I]
     L.omx_tune_query.argtypes = [I, I, I, I, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), D]
     L.omx_tune_reset.argtypes = []
+    L.omx_set_graphs.argtypes = [P, I]
     _lib = L
     return L
 
@@ -158,6 +160,10 @@ class Context:
     def trim(self):
         """Release the matrix-sized device buffers."""
         _check(self.L.omx_trim(self.h))
+
+    def set_graphs(self, on):
+        """Factorisations captured in HIP graphs once per state and replayed (default off)."""
+        _check(self.L.omx_set_graphs(self.h, 1 if on else 0))
 
     def set_autotune(self, on):
         """Batched sweeps tune their products and triangular solves per shape (default on)."""

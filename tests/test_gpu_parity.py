@@ -103,7 +103,8 @@ def test_fundamental_solve(ctx, ora, name, kind, tol):
         assert abs(cs[k] - ref[k]) <= tol * abs(ref[k]), k
 
 
-@pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL), ("gmres32ir", MIXED_TOL)])
+@pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL), ("gmres32ir", MIXED_TOL),
+                                      ("auto", MIXED_TOL)])
 def test_second_harmonic_chain(ctx, ora, kind, tol):
     c = small_drude(P=4, nmax=3)
     ref = ora.solve_all(c)
@@ -122,7 +123,7 @@ def test_second_harmonic_chain(ctx, ora, kind, tol):
     assert abs(cs["cabs_sh"] - own) <= 1e-12 * abs(own)
 
 
-@pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL)])
+@pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL), ("auto", MIXED_TOL)])
 def test_nmax9_cluster(ctx, ora, kind, tol):
     """nmax 9 (pair_kernel<9>): three Drude spheres, omega and 2 omega; S, V
     and Q to 1e-12, x_sca, x_sca_sh and the cross sections at the bar."""

@@ -3,7 +3,8 @@
 
 At N >= 16384 the automatic kind starts with the mixed Krylov; with an
 iteration budget too small to converge it must fall back to the mixed LU on
-the same matrices and return the LU's answer.  Every path is held to the
+the same matrices and return the LU's answer.  Below that, with the second
+harmonic, S is tried with FP64 GMRES while V's mixed LU factorises.  Every path is held to the
 mixed bar (1e-9) against the mixed LU of the same system, whose own parity
 with the oracle is tested at smaller sizes (tests/test_gpu_parity.py) and at
 config 4 by residual (tests/test_gpu_configs.py)."""
@@ -73,11 +74,45 @@ def test_krylov_paths_at_size(big, reference, kind):
 
 
 def test_auto_small_system_uses_the_lu():
-    """Below N = 16384 the automatic kind is the mixed LU: no Krylov products
-    beyond the refinement's, and the LU's answer to rounding."""
-    c = drude_cluster(8, 5, 700e-9, 40, 80, 600, 10, seed=11, sh=True)
+    """Below N = 16384, without the second harmonic, the automatic kind is the
+    mixed LU: the LU's iterations and its answer to rounding."""
+    c = drude_cluster(8, 5, 700e-9, 40, 80, 600, 10, seed=11, sh=False)
     out_a, cs_a, st_a = run(c, "auto")
     out_l, cs_l, st_l = run(c, "lu32ir")
     assert st_a["iterations"] == st_l["iterations"]
     assert rel(out_a["x_sca"], out_l["x_sca"]) < 1e-14
-    assert rel(out_a["x_sca_sh"], out_l["x_sca_sh"]) < 1e-13
+
+
+def test_auto_small_sh_system_tries_gmres_for_s():
+    """With the second harmonic, V's mixed LU starts at once and S is solved by
+    FP64 GMRES beside it: S takes exactly the iterations of the FP64 GMRES kind
+    (same restart, tolerance and zero start), V about those of the mixed LU, and both
+    answers agree with the mixed LU's to the mixed bar."""
+    c = drude_cluster(8, 5, 700e-9, 40, 80, 600, 10, seed=11, sh=True)
+    print(f"auto with SH: N {c.N}", flush=True)
+    out_a, cs_a, st_a = run(c, "auto")
+    out_l, cs_l, st_l = run(c, "lu32ir")
+    out_g, cs_g, st_g = run(c, "gmres64")
+    assert st_a["pivot_fallbacks"] == 0
+    assert st_a["iterations"] == st_g["iterations"]
+    # V's right-hand side comes from the fundamental solution, which differs
+    # from the LU's at rounding level: its refinement at tol 1e-16 may take
+    # one step more or less
+    assert abs(st_a["iterations_sh"] - st_l["iterations_sh"]) <= 1
+    for k in ("x_sca", "x_int", "x_sca_sh", "x_int_sh"):
+        assert rel(out_a[k], out_l[k]) < 1e-10, k
+    assert rel(out_a["x_sca"], out_g["x_sca"]) < 1e-14
+    for k in ("cext", "csca", "cabs", "csca_sh", "cabs_sh"):
+        assert cs_a[k] == pytest.approx(cs_l[k], rel=1e-10), k
+
+
+def test_auto_small_sh_system_falls_back_to_the_lu():
+    """One GMRES step cannot reach FP64 accuracy: S is factored after all and
+    the answer is the mixed LU's."""
+    c = drude_cluster(8, 5, 700e-9, 40, 80, 600, 10, seed=11, sh=True)
+    out_a, cs_a, st_a = run(c, "auto", budget=1)
+    out_l, cs_l, st_l = run(c, "lu32ir")
+    assert st_a["pivot_fallbacks"] == 1
+    assert st_a["iterations"] == st_l["iterations"] and st_a["iterations_sh"] == st_l["iterations_sh"]
+    for k in ("x_sca", "x_sca_sh"):
+        assert rel(out_a[k], out_l[k]) < 1e-13, k
