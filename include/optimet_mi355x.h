@@ -66,7 +66,9 @@ extern "C" {
 #define OMX_SOLVER_AUTO 5      /* by size: N >= 16384 starts with OMX_SOLVER_GMRES32_IR within a
                                   budget of the mixed LU's cost and falls back to
                                   OMX_SOLVER_LU32_IR on the same matrices; smaller systems use
-                                  OMX_SOLVER_LU32_IR (sweeps always batch the LU) */
+                                  OMX_SOLVER_LU32_IR, except that with the second harmonic S
+                                  is first tried with FP64 GMRES (within N / 125 products)
+                                  while V's mixed LU factorises (sweeps always batch the LU) */
 
 /* Pivot search of the LU factorisations (omx_set_pivoting). */
 #define OMX_PIVOT_BLOCK 0     /* among the rows of each 64 x 64 diagonal block, then
@@ -140,10 +142,11 @@ int omx_set_excitation(omx_context *ctx, double wavelength, double theta, double
  * GMRES restart length (<= 256). */
 int omx_set_solver(omx_context *ctx, int kind, double tol, int maxit, int restart);
 
-/* The Krylov budget of OMX_SOLVER_AUTO in outer steps (each at most 31
- * products); 0 (default) takes the cost model: 0.0143 N products, what the
- * mixed LU would cost.  No reference counterpart: a tuning knob of the
- * automatic choice. */
+/* The Krylov budget of OMX_SOLVER_AUTO: outer steps of the mixed Krylov (each
+ * at most 31 products) from N = 16384 on, FP64 GMRES products for S below it;
+ * 0 (default) takes the cost model, what the mixed LU would cost (0.0143 N
+ * FP32 products, N / 125 FP64 products).  No reference counterpart: a tuning
+ * knob of the automatic choice. */
 int omx_set_auto_budget(omx_context *ctx, int steps);
 
 /* Autotuning of the batched sweeps (default on; no reference counterpart): per
