@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <tuple>
 #include <stdexcept>
 #include <string>
@@ -1019,7 +1020,6 @@ void solve_system(omx_context *c, bool shf, const cplx *rhs, cplx *x, int *iters
     HIPCHK(hipMemsetAsync(x, 0, sizeof(cplx) * Np, st));
     const int maxit = std::min(c->maxit, c->auto_budget > 0 ? c->auto_budget : auto_gmres64_steps(c->N));
     *iters = gm.solve(rhs, x, c->tol, maxit, c->restart, res);
-    check_status(c, shf);
     if(*res <= 1e3 * c->tol || backward_stable(c, gm, A64, rhs, x, *res, st)) return;
     ++c->fallbacks;
   }
@@ -1956,6 +1956,7 @@ void omx_destroy(omx_context *c) {
     if(e) (void)hipEventDestroy(e);
   for(auto e : c->xev)
     if(e) (void)hipEventDestroy(e);
+
   for(auto e : c->pev2)
     if(e) (void)hipEventDestroy(e);
   if(c->aux2) (void)hipStreamDestroy(c->aux2);
@@ -2215,6 +2216,37 @@ int omx_update(omx_context *c) {
   });
 }
 
+// Enqueues V's factorisation from a host thread of its own: the enqueueing
+// of a factorisation proceeds at the device's pace, so enqueued from the
+// calling thread it would hold back S's iterative solve (which needs the host
+// between its products) until V's factorisation had run.  join() rethrows an
+// error of the thread; the destructor joins without rethrowing.
+struct SideEnqueue {
+  std::thread t;
+  std::exception_ptr err;
+  void start(omx_context *c, void (*f)(void *)) {
+    t = std::thread([this, c, f] {
+      try {
+        HIPCHK(hipSetDevice(c->device));
+        f(c);
+      } catch(...) {
+        err = std::current_exception();
+      }
+    });
+  }
+  void join() {
+    if(t.joinable()) t.join();
+    if(err) {
+      std::exception_ptr e = err;
+      err = nullptr;
+      std::rethrow_exception(e);
+    }
+  }
+  ~SideEnqueue() {
+    if(t.joinable()) t.join();
+  }
+};
+
 int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, double *x_int_sh) {
   return guard([&] {
     if(!c || !c->updated) throw OmxError(OMX_ERR_STATE, "omx_update must precede omx_solve");
@@ -2229,6 +2261,7 @@ int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, do
     const bool lu_kind = is_lu(c->kind);
     const bool graphed =
         c->graphs && lu_kind && !c->profile && c->pivot == OMX_PIVOT_BLOCK && !c->factored && !c->krylov_s;
+    SideEnqueue vq;
     if(c->sh && lu_kind && !c->factored_sh && !c->profile) {
       // V's factorisation starts once S's has passed a fraction of its
       // columns (by then S's trailing updates no longer fill the device),
@@ -2258,6 +2291,9 @@ int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, do
         }
         c->lu.ev_mark = nullptr;
         c->lu.on_mark = nullptr;
+      } else if(c->krylov_s && !c->factored) {
+        HIPCHK(hipEventRecord(c->xev[0], c->st));
+        vq.start(c, start_v);  // S's GMRES starts at once beside it
       } else {
         HIPCHK(hipEventRecord(c->xev[0], c->st));
         start_v(c);
@@ -2288,7 +2324,9 @@ int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, do
       omx::run_shsrc(sa, c->st);
       HIPCHK(hipEventRecord(c->ev[4], c->st));
       // the second-harmonic solve runs on the second stream, after the source
+      // (and after V's factorisation has been enqueued there)
       HIPCHK(hipEventRecord(c->xev[1], c->st));
+      vq.join();
       HIPCHK(hipStreamWaitEvent(c->st2, c->xev[1], 0));
       solve_system(c, true, c->d_K.as<cplx>(), c->d_xsh.as<cplx>(), &c->stats.iterations_sh, &c->stats.residual_sh);
       // X_int_SH = X_sca_SH (.) IauxSH1 - K1ana (Solver.cpp:95-116)
