@@ -532,9 +532,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
 #pragma unroll
   for(int m = 0; m < 16; ++m) x[m] = csel(16 * w + m == i, one, zero);
   if(q < 4) {
-    // L11^{-1} (unit lower): for k: row i -= L[i][k] row k, i > k
+    // L11^{-1} (unit lower): for k: row i -= L[i][k] row k, i > k.  Column
+    // 16 w + m starts as e_{16 w + m} and its rows above 16 w + m stay zero,
+    // so the steps k < 16 w leave this wave's columns as they are: skipped
 #pragma unroll 1
-    for(int k = 0; k < 63; ++k) {
+    for(int k = 16 * w; k < 63; ++k) {
       const int ks = __builtin_amdgcn_readfirstlane(k);
       const C lik = csel(i > ks, F[ks * 65 + i], zero);
 #pragma unroll
@@ -543,11 +545,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   } else {
     // U11^{-1} = (D^{-1} U11)^{-1} D^{-1} with D = diag(U11): a unit-upper
     // sweep (row i -= (U[i][k] / U[i][i]) row k, i < k, k = 63..0), then
-    // column j scaled by 1 / U[j][j]
+    // column j scaled by 1 / U[j][j].  Rows below 16 w + 15 of this wave's
+    // columns stay zero, so the sweep starts at k = 16 w + 15
     const C di = F[i * 65 + i];
     const C rinv = csel(mag2(di) > 0.0, crecip(di), zero);
 #pragma unroll 1
-    for(int k = 63; k > 0; --k) {
+    for(int k = 16 * w + 15; k > 0; --k) {
       const int ks = __builtin_amdgcn_readfirstlane(k);
       const C uik = csel(i < ks, cm(F[ks * 65 + i], rinv), zero);
 #pragma unroll
