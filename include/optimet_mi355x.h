@@ -167,6 +167,24 @@ int omx_tune_query(int device, int Np, int nb, int fp64, int *split, int *trsv, 
 /* Forget every tuned choice of this process. */
 int omx_tune_reset(void);
 
+/* No reference counterpart: continuous batching across requests.  Solves the
+ * n member contexts, each with its own geometry and excitation and already
+ * assembled by omx_update (clusters of any P and nmax; one solver kind for all,
+ * OMX_SOLVER_LU64 or OMX_SOLVER_LU32_IR, and one SH flag), as one batch:
+ * their S (and V) are bordered by the identity to the largest padded order,
+ * factored by one batched launch sequence and refined in lockstep in the
+ * leader's sweep buffers; each member's x_int, SH source and x_int_SH come from
+ * its own kernels.  Afterwards every member holds its solution as after
+ * omx_solve (omx_read_solution, omx_cross_sections, omx_fields).  A member
+ * whose block pivots or refinement fall short is solved alone by omx_solve.
+ * The leader needs no geometry, and its own settings are left as they were.
+ * All contexts must be on the leader's device and distinct from it. */
+int omx_solve_group(omx_context *leader, int n, omx_context *const *members);
+
+/* The solution of the last omx_solve or omx_solve_group (each pointer may be
+ * NULL; N complex values each, interleaved). */
+int omx_read_solution(omx_context *ctx, double *x_sca, double *x_int, double *x_sca_sh, double *x_int_sh);
+
 /* No reference counterpart.  On: the block-pivoting factorisations of
  * omx_solve and of omx_solve_sweep's chunks are captured in HIP graphs once
  * per state (order, batch, solver kind, product, buffers) and replayed;

@@ -2216,6 +2216,27 @@ int omx_update(omx_context *c) {
   });
 }
 
+// The nine Clebsch-Gordan-type tables of the SH source for the context's
+// nmax (Symbol.cpp:1036-1451), built once per nmax.
+void ensure_cg(omx_context *c, hipStream_t st) {
+  if(c->cg_nmax == c->nmax) return;
+  size_t nf = (size_t)c->nf, sz = nf * nf * nf;
+  c->d_cg.ensure(sizeof(double) * 9 * sz, &c->total);
+  omx::run_cg(c->nmax, c->d_cg.as<double>(), st);
+  c->cg_nmax = c->nmax;
+}
+
+// The second-harmonic source K and K1 of the context's x_int
+// (PreconditionedMatrix.cpp:1347-1436, Geometry.cpp:250-310), on stream st.
+void sh_source(omx_context *c, hipStream_t st) {
+  HIPCHK(hipMemsetAsync(c->d_K.p, 0, sizeof(cplx) * c->Np, st));
+  omx::ShArgs sa{c->P, c->nmax, c->omega(), kEps0 * c->bg_eps_r, kMu0 * c->bg_mu_r, c->d_radius.as<double>(),
+                 c->d_eps.as<cplx>(), c->d_mu.as<cplx>(), c->d_eps_sh.as<cplx>(), c->d_kppp.as<cplx>(),
+                 c->d_kpar.as<cplx>(), c->d_gam.as<cplx>(), c->d_ov.as<cplx>(), c->d_cg.as<double>(),
+                 c->d_xint.as<cplx>(), c->d_K.as<cplx>(), c->d_K1.as<cplx>()};
+  omx::run_shsrc(sa, st);
+}
+
 // Enqueues V's factorisation from a host thread of its own: the enqueueing
 // of a factorisation proceeds at the device's pace, so enqueued from the
 // calling thread it would hold back S's iterative solve (which needs the host
@@ -2309,19 +2330,9 @@ int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, do
     HIPCHK(hipEventSynchronize(c->ev[2]));
     ts += elapsed(c, 0, 2);
     if(c->sh) {
-      if(c->cg_nmax != nmax) {
-        size_t nf = (size_t)c->nf, sz = nf * nf * nf;
-        c->d_cg.ensure(sizeof(double) * 9 * sz, &c->total);
-        omx::run_cg(nmax, c->d_cg.as<double>(), c->st);
-        c->cg_nmax = nmax;
-      }
+      ensure_cg(c, c->st);
       HIPCHK(hipEventRecord(c->ev[3], c->st));
-      HIPCHK(hipMemsetAsync(c->d_K.p, 0, sizeof(cplx) * c->Np, c->st));
-      omx::ShArgs sa{P, nmax, c->omega(), kEps0 * c->bg_eps_r, kMu0 * c->bg_mu_r, c->d_radius.as<double>(),
-                     c->d_eps.as<cplx>(), c->d_mu.as<cplx>(), c->d_eps_sh.as<cplx>(), c->d_kppp.as<cplx>(),
-                     c->d_kpar.as<cplx>(), c->d_gam.as<cplx>(), c->d_ov.as<cplx>(), c->d_cg.as<double>(),
-                     c->d_xint.as<cplx>(), c->d_K.as<cplx>(), c->d_K1.as<cplx>()};
-      omx::run_shsrc(sa, c->st);
+      sh_source(c, c->st);
       HIPCHK(hipEventRecord(c->ev[4], c->st));
       // the second-harmonic solve runs on the second stream, after the source
       // (and after V's factorisation has been enqueued there)
@@ -2354,6 +2365,229 @@ int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, do
     }
     HIPCHK(hipStreamSynchronize(c->st));
     c->solved = true;
+  });
+}
+
+// ------------------------------------------------------------------ groups
+// Batch buffers of a group solve in the leader context: the sweep's buffers
+// that the batched factorisation, triangular solves and lockstep refinement
+// use, at the group's padded order L->Np.
+void group_alloc(omx_context *L, int nb, bool f64, bool sh) {
+  auto &w = L->sw;
+  size_t *T = &L->total;
+  const size_t Np = L->Np, n2 = Np * Np;
+  const int m = L->restart;
+  w.S64.ensure(sizeof(cplx) * n2 * nb, T);
+  if(f64) w.F64.ensure(sizeof(cplx) * n2 * nb, T); else w.S32.ensure(sizeof(float2) * n2 * nb, T);
+  if(sh) {
+    w.V64.ensure(sizeof(cplx) * n2 * nb, T);
+    if(f64) w.F64sh.ensure(sizeof(cplx) * n2 * nb, T); else w.V32.ensure(sizeof(float2) * n2 * nb, T);
+  }
+  for(DBuf *b : {&w.Q, &w.x, &w.xsh, &w.K, &w.r, &w.w1, &w.w2}) b->ensure(sizeof(cplx) * Np * nb, T);
+  for(DBuf *b : {&w.perm, &w.perm2}) b->ensure(sizeof(int) * Np * nb, T);
+  for(DBuf *b : {&w.flags, &w.flags2}) b->ensure(sizeof(int) * (Np / omx::kNB + 1) * nb, T);
+  for(DBuf *b : {&w.dinv, &w.dinv2}) b->ensure(sizeof(cplx) * omx::dinv_elems(Np) * nb, T);
+  for(DBuf *b : {&w.map, &w.map2}) b->ensure(sizeof(int) * 3 * (Np + 2 * omx::kNB) * nb, T);
+  for(DBuf *b : {&w.status, &w.status2}) b->ensure(sizeof(int) * 4 * nb, T);
+  w.sys.ensure(sizeof(int) * nb, T);
+  w.basis.ensure(sizeof(cplx) * Np * (m + 1) * nb, T);
+  w.basisZ.ensure(sizeof(cplx) * Np * (m + 1) * nb, T);
+  w.gpart.ensure(sizeof(cplx) * Np * omx::kGemvSplitMax * nb, T);
+  w.part.ensure(sizeof(cplx) * 128 * (m + 2) * nb, T);
+  w.hd.ensure(sizeof(cplx) * (m + 2) * nb, T);
+  w.nrm.ensure(sizeof(double) * 160 * nb, T);
+  w.sc.ensure(sizeof(cplx) * 2 * nb, T);
+  w.act.ensure(sizeof(int) * nb, T);
+  L->d_status.ensure(64, T);
+}
+
+int omx_solve_group(omx_context *L, int n, omx_context *const *mem) {
+  return guard([&] {
+    if(!L || n < 1 || !mem) throw OmxError(OMX_ERR_ARG, "omx_solve_group: leader, count and members required");
+    for(int i = 0; i < n; ++i) {
+      omx_context *m = mem[i];
+      if(!m || m == L) throw OmxError(OMX_ERR_ARG, "omx_solve_group: member " + std::to_string(i) + " invalid");
+      if(!m->updated) throw OmxError(OMX_ERR_STATE, "omx_solve_group: member " + std::to_string(i) + " not updated");
+      if(m->device != L->device) throw OmxError(OMX_ERR_ARG, "omx_solve_group: members must share the leader's device");
+      if(m->kind != mem[0]->kind || (m->kind != OMX_SOLVER_LU64 && m->kind != OMX_SOLVER_LU32_IR))
+        throw OmxError(OMX_ERR_ARG, "omx_solve_group: members need one LU kind (lu64 or lu32ir)");
+      if((m->sh != 0) != (mem[0]->sh != 0)) throw OmxError(OMX_ERR_ARG, "omx_solve_group: mixed SH flags");
+      for(int j = 0; j < i; ++j)
+        if(mem[j] == m) throw OmxError(OMX_ERR_ARG, "omx_solve_group: a member appears twice");
+    }
+    HIPCHK(hipSetDevice(L->device));
+    const bool f64 = mem[0]->kind == OMX_SOLVER_LU64, sh = mem[0]->sh != 0;
+    int Np = 0;
+    for(int i = 0; i < n; ++i) Np = std::max(Np, mem[i]->Np);
+    // the leader runs the batch at the largest padded order with the first
+    // member's refinement settings, in its sweep buffers only; its own
+    // sizes and settings come back on every exit
+    struct Restore {
+      omx_context &c;
+      int N, Np, kind, sh, maxit, restart, gemm;
+      double tol;
+      ~Restore() {
+        c.N = N;
+        c.Np = Np;
+        c.kind = kind;
+        c.sh = sh;
+        c.maxit = maxit;
+        c.restart = restart;
+        c.lu32_gemm = gemm;
+        c.tol = tol;
+      }
+    } restore{*L, L->N, L->Np, L->kind, L->sh, L->maxit, L->restart, L->lu32_gemm, L->tol};
+    L->Np = Np;
+    L->N = Np;
+    L->kind = mem[0]->kind;
+    L->sh = sh;
+    L->tol = mem[0]->tol;
+    L->maxit = mem[0]->maxit;
+    L->restart = mem[0]->restart;
+    L->lu32_gemm = mem[0]->lu32_gemm;
+    const int nb = n;
+    const size_t n2 = (size_t)Np * Np, lda = (size_t)Np;
+    group_alloc(L, nb, f64, sh);
+    const TuneChoice tune = L->autotune ? sweep_tune(L, nb, f64) : TuneChoice{L->lu32_gemm == OMX_GEMM_BF16X3, 0};
+    auto &w = L->sw;
+    hipStream_t st = L->st;
+    for(DBuf *b : {&w.flags, &w.flags2, &w.status, &w.status2, &w.basisZ, &w.w1}) HIPCHK(hipMemsetAsync(b->p, 0, b->bytes, st));
+    HIPCHK(hipMemsetAsync(L->d_status.p, 0, 16, st));
+    // the members' matrices and right-hand sides, bordered by the identity
+    cplx *S64 = w.S64.as<cplx>(), *V64 = sh ? w.V64.as<cplx>() : nullptr;
+    for(int z = 0; z < nb; ++z) {
+      omx_context *m = mem[z];
+      omx::pad_matrix64(S64 + z * n2, Np, m->d_S64.as<cplx>(), m->Np, st);
+      if(f64) omx::pad_matrix64(w.F64.as<cplx>() + z * n2, Np, m->d_S64.as<cplx>(), m->Np, st);
+      else omx::pad_matrix32(w.S32.as<float2>() + z * n2, Np, m->d_S64.as<cplx>(), m->Np, st);
+      if(sh) {
+        omx::pad_matrix64(V64 + z * n2, Np, m->d_V64.as<cplx>(), m->Np, st);
+        if(f64) omx::pad_matrix64(w.F64sh.as<cplx>() + z * n2, Np, m->d_V64.as<cplx>(), m->Np, st);
+        else omx::pad_matrix32(w.V32.as<float2>() + z * n2, Np, m->d_V64.as<cplx>(), m->Np, st);
+      }
+      omx::pad_vector(w.Q.as<cplx>() + z * lda, Np, m->d_Q.as<cplx>(), m->N, st);
+    }
+    HIPCHK(hipGetLastError());
+    void *FS = f64 ? w.F64.p : w.S32.p, *FV = sh ? (f64 ? w.F64sh.p : w.V32.p) : nullptr;
+    auto factor = [&](void *F, omx::LuWork &lw, hipStream_t fst) {
+      const int rc = f64 ? omx::lu_factor<double2>(static_cast<double2 *>(F), Np, lda, lw, fst, nullptr)
+                         : omx::lu_factor<float2>(static_cast<float2 *>(F), Np, lda, lw, fst, nullptr);
+      if(rc) throw OmxError(OMX_ERR_DEVICE, "batched lu_factor launch failed (" + std::to_string(rc) + ")");
+    };
+    omx::LuWork lwS = sweep_lu(L, nb, false), lwV{};
+    lwS.split = tune.split;
+    lwS.trsv = tune.trsv;
+    if(sh) {
+      lwV = sweep_lu(L, nb, true);
+      lwV.split = tune.split;
+      lwV.trsv = tune.trsv;
+      HIPCHK(hipEventRecord(L->xev[0], st));
+      HIPCHK(hipStreamWaitEvent(L->st2, L->xev[0], 0));
+      factor(FV, lwV, L->st2);
+      HIPCHK(hipEventRecord(L->xev[1], L->st2));
+    }
+    factor(FS, lwS, st);
+    std::vector<int> bad(nb, 0), iters(2 * nb, 0);
+    std::vector<double> resid(2 * nb, 0.0);
+    auto status_scan = [&](bool second) {
+      std::vector<int> h(4 * nb);
+      HIPCHK(hipMemcpyAsync(h.data(), (second ? w.status2 : w.status).p, sizeof(int) * 4 * nb,
+                            hipMemcpyDeviceToHost, st));
+      int g[2];
+      HIPCHK(hipMemcpyAsync(g, L->d_status.p, sizeof(g), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      if(h[1] || g[1]) throw OmxError(OMX_ERR_DEVICE, "device wait timed out in a group solve");
+      for(int z = 0; z < nb; ++z)
+        if(h[4 * z]) bad[z] = 1;
+    };
+    auto refine = [&](bool second, omx::LuWork &lw, void *F, const cplx *A64, const cplx *rhs, cplx *x) {
+      status_scan(second);
+      omx::LuWork *lwp = &lw;
+      GmresB gm{L, A64, n2, Np, nb, nullptr, w};
+      gm.lw = lwp;
+      if(f64)
+        gm.prec = [L, F, lwp, Np, lda](const cplx *in, cplx *out) {
+          omx::lu_solve<double2>(static_cast<const double2 *>(F), Np, lda, *lwp, in, out, L->st);
+        };
+      else
+        gm.prec = [L, F, lwp, Np, lda](const cplx *in, cplx *out) {
+          omx::lu_solve<float2>(static_cast<const float2 *>(F), Np, lda, *lwp, in, out, L->st);
+        };
+      gm.prec(rhs, x);
+      std::vector<double> res;
+      std::vector<int> its;
+      const int bmax = nb > 1 ? std::min(L->maxit, 4 * L->restart) : L->maxit;
+      gm.solve(rhs, x, L->tol, bmax, L->restart, res, its);
+      status_scan(second);
+      for(int z = 0; z < nb; ++z) {
+        if(!(res[z] <= 1e3 * L->tol)) bad[z] = 1;
+        iters[2 * z + (second ? 1 : 0)] = its[z];
+        resid[2 * z + (second ? 1 : 0)] = res[z];
+      }
+    };
+    // fundamental: S x = Q for every member, then each member's x and x_int
+    refine(false, lwS, FS, S64, w.Q.as<cplx>(), w.x.as<cplx>());
+    for(int z = 0; z < nb; ++z) {
+      omx_context *m = mem[z];
+      HIPCHK(hipMemcpyAsync(m->d_x.p, w.x.as<cplx>() + z * lda, sizeof(cplx) * m->Np, hipMemcpyDeviceToDevice, st));
+      omx::vmul_ov(m->d_xint.as<cplx>(), m->d_x.as<cplx>(), m->d_ov.as<cplx>(), omx::OV_IAUX, m->P, m->nmax, nullptr,
+                   m->N, st);
+    }
+    if(sh) {
+      // each member's SH source from its own x_int, then V x = K in one batch
+      for(int z = 0; z < nb; ++z) {
+        omx_context *m = mem[z];
+        ensure_cg(m, st);
+        sh_source(m, st);
+        omx::pad_vector(w.K.as<cplx>() + z * lda, Np, m->d_K.as<cplx>(), m->N, st);
+      }
+      HIPCHK(hipStreamWaitEvent(st, L->xev[1], 0));  // V's factorisation
+      refine(true, lwV, FV, V64, w.K.as<cplx>(), w.xsh.as<cplx>());
+      for(int z = 0; z < nb; ++z) {
+        omx_context *m = mem[z];
+        HIPCHK(hipMemcpyAsync(m->d_xsh.p, w.xsh.as<cplx>() + z * lda, sizeof(cplx) * m->Np, hipMemcpyDeviceToDevice,
+                              st));
+        omx::vmul_ov(m->d_xintsh.as<cplx>(), m->d_xsh.as<cplx>(), m->d_ov.as<cplx>(), omx::OV_IAUXSH1, m->P, m->nmax,
+                     m->d_K1.as<cplx>(), m->N, st);
+      }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipStreamSynchronize(L->st2));
+    for(int z = 0; z < nb; ++z) {
+      omx_context *m = mem[z];
+      if(bad[z]) {
+        // zero pivot in the block or a refinement short of the target: this
+        // member alone, with the single-system path and its fallbacks
+        if(omx_solve(m, nullptr, nullptr, nullptr, nullptr) != OMX_OK)
+          throw OmxError(OMX_ERR_NOCONV, "omx_solve_group: member " + std::to_string(z) + ": " + omx_last_error());
+        ++L->fallbacks;
+        continue;
+      }
+      m->stats.iterations = iters[2 * z];
+      m->stats.iterations_sh = iters[2 * z + 1];
+      m->stats.residual = resid[2 * z];
+      m->stats.residual_sh = resid[2 * z + 1];
+      m->solved = true;
+    }
+    L->stats.sweep_batch = nb;
+  });
+}
+
+int omx_read_solution(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, double *x_int_sh) {
+  return guard([&] {
+    if(!c || !c->solved) throw OmxError(OMX_ERR_STATE, "no solution: omx_solve or omx_solve_group first");
+    HIPCHK(hipSetDevice(c->device));
+    auto cp = [&](double *dst, const DBuf &b) {
+      if(dst) HIPCHK(hipMemcpyAsync(dst, b.p, sizeof(cplx) * c->N, hipMemcpyDeviceToHost, c->st));
+    };
+    cp(x_sca, c->d_x);
+    cp(x_int, c->d_xint);
+    if(c->sh) {
+      cp(x_sca_sh, c->d_xsh);
+      cp(x_int_sh, c->d_xintsh);
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
   });
 }
 

@@ -1,5 +1,7 @@
 // omx_iter.hip -- level-1 kernels of the iterative solvers (GMRES, GMRES-IR)
 // and the element-wise unpreconditioning steps (Solver.cpp:57-116).
+#include <type_traits>
+
 #include "omx_kernels.hpp"
 
 namespace omx {
@@ -115,6 +117,35 @@ __global__ void zero_ints_k(int *p, int n) {
   if(i < n) p[i] = 0;
 }
 void zero_ints(int *p, int n, hipStream_t st) { hipLaunchKernelGGL(zero_ints_k, g1(n), dim3(256), 0, st, p, n); }
+
+// dst (Nd x Nd, column-major) = src (Ns x Ns, leading dimension Ns) bordered
+// by the identity (grid-stride: Nd^2 may exceed 2^31)
+template <typename D>
+__global__ void pad_matrix_k(D *dst, int Nd, const cplx *src, int Ns) {
+  const size_t n = (size_t)Nd * Nd;
+  for(size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e / Nd), i = (int)(e % Nd);
+    const cplx v = (i < Ns && j < Ns) ? src[(size_t)j * Ns + i] : mk(i == j ? 1.0 : 0.0);
+    if constexpr(std::is_same<D, cplx>::value)
+      dst[e] = v;
+    else
+      dst[e] = make_float2((float)v.x, (float)v.y);
+  }
+}
+void pad_matrix64(cplx *dst, int Nd, const cplx *src, int Ns, hipStream_t st) {
+  hipLaunchKernelGGL(pad_matrix_k<cplx>, dim3(4096), dim3(256), 0, st, dst, Nd, src, Ns);
+}
+void pad_matrix32(float2 *dst, int Nd, const cplx *src, int Ns, hipStream_t st) {
+  hipLaunchKernelGGL(pad_matrix_k<float2>, dim3(4096), dim3(256), 0, st, dst, Nd, src, Ns);
+}
+// dst[0, Nd) = src[0, Ns) followed by zeros
+__global__ void pad_vector_k(cplx *dst, int Nd, const cplx *src, int Ns) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < Nd) dst[i] = i < Ns ? src[i] : mk(0.0);
+}
+void pad_vector(cplx *dst, int Nd, const cplx *src, int Ns, hipStream_t st) {
+  hipLaunchKernelGGL(pad_vector_k, g1(Nd), dim3(256), 0, st, dst, Nd, src, Ns);
+}
 void vcopy(cplx *y, const cplx *x, int n, hipStream_t st) { hipLaunchKernelGGL(vcopy_k, g1(n), dim3(256), 0, st, y, x, n); }
 void vaxpby(cplx *y, cplx a, const cplx *x, cplx b, int n, hipStream_t st) {
   hipLaunchKernelGGL(vaxpby_k, g1(n), dim3(256), 0, st, y, a, x, b, n);

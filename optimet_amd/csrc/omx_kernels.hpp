@@ -237,6 +237,11 @@ void vcopy(cplx *y, const cplx *x, int n, hipStream_t st);
 void demote(float2 *y, const cplx *x, size_t n, hipStream_t st);                 // y = (c64) x
 // p[0..n) = 0 by a kernel (a memset node must not go into a captured graph)
 void zero_ints(int *p, int n, hipStream_t st);
+// dst (Nd x Nd) = src (Ns x Ns, Ns <= Nd) bordered by the identity, in FP64 or
+// rounded to complex FP32; dst[0, Nd) = src[0, Ns) followed by zeros
+void pad_matrix64(cplx *dst, int Nd, const cplx *src, int Ns, hipStream_t st);
+void pad_matrix32(float2 *dst, int Nd, const cplx *src, int Ns, hipStream_t st);
+void pad_vector(cplx *dst, int Nd, const cplx *src, int Ns, hipStream_t st);
 void vaxpby(cplx *y, cplx a, const cplx *x, cplx b, int n, hipStream_t st);   // y = a x + b y
 void vsub(cplx *z, const cplx *x, const cplx *y, int n, hipStream_t st);      // z = x - y
 void vmul_ov(cplx *y, const cplx *x, const cplx *ov, int kind, int P, int nmax, const cplx *sub, int n,

@@ -34,7 +34,7 @@ EXPORTS = ["omx_last_error", "omx_create", "omx_destroy", "omx_set_geometry", "o
            "omx_dense_solve", "omx_set_profile", "omx_set_pivoting", "omx_set_lu32_gemm", "omx_solve_sweep", "omx_mem_info", "omx_trim",
            "omx_dense_solve_noalloc", "omx_read_matrix_block", "omx_absorption_cross_section_sh",
            "omx_fields", "omx_aca_info", "omx_aca_block", "omx_set_auto_budget", "omx_set_autotune",
-           "omx_tune_query", "omx_tune_reset", "omx_set_graphs"]
+           "omx_tune_query", "omx_tune_reset", "omx_set_graphs", "omx_solve_group", "omx_read_solution"]
 
 
 class OmxError(RuntimeError):
@@ -106,6 +106,8 @@ def load_library(path=LIB_PATH):
     L.omx_tune_query.argtypes = [I, I, I, I, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), D]
     L.omx_tune_reset.argtypes = []
     L.omx_set_graphs.argtypes = [P, I]
+    L.omx_solve_group.argtypes = [P, I, ctypes.POINTER(ctypes.c_void_p)]
+    L.omx_read_solution.argtypes = [P, D, D, D, D]
     _lib = L
     return L
 
@@ -269,6 +271,16 @@ class Solver:
             out.update({"x_sca_sh": xss, "x_int_sh": xis})
         return out
 
+    def read_solution(self):
+        """The solution of the last solve or group solve, as solve() returns it."""
+        N = self.cluster.N
+        out = {k: np.zeros(N, np.complex128) for k in ("x_sca", "x_int")}
+        if self.cluster.sh:
+            out.update({k: np.zeros(N, np.complex128) for k in ("x_sca_sh", "x_int_sh")})
+        p = [_d(out[k].view(np.float64)) if k in out else None for k in ("x_sca", "x_int", "x_sca_sh", "x_int_sh")]
+        _check(self.ctx.L.omx_read_solution(self.ctx.h, *p))
+        return out
+
     def sweep(self, wavelengths, batch=0, outputs=("x_sca", "x_int", "x_sca_sh", "x_int_sh"), eps=None,
               eps_sh=None):
         """Batched wavelength sweep (omx_solve_sweep): the cluster at every
@@ -397,3 +409,23 @@ def tune_query(Np, nb, fp64=False, device=0):
 def tune_reset():
     """Forget the tuned choices of this process."""
     load_library().omx_tune_reset()
+
+
+def solve_group(solvers, leader=None):
+    """omx_solve_group: the solvers (each with its own context, updated, one
+    LU kind, one SH flag; clusters of any size) solved as one batch in the
+    leader context (a new one when None).  Returns each member's solution."""
+    solvers = list(solvers)
+    if not solvers:
+        return []
+    ctxs = [s.ctx for s in solvers]
+    if len({id(c) for c in ctxs}) != len(ctxs):
+        raise ValueError("solve_group: every member needs a context of its own")
+    L = leader or Context()
+    try:
+        arr = (ctypes.c_void_p * len(ctxs))(*[c.h.value for c in ctxs])
+        _check(L.L.omx_solve_group(L.h, len(ctxs), arr))
+    finally:
+        if leader is None:
+            L.close()
+    return [s.read_solution() for s in solvers]
