@@ -308,6 +308,96 @@ def config4_main(args):
     print(json.dumps(line), flush=True)
 
 
+def group_main(args):
+    """Continuous batching of small uneven requests (omx_solve_group): 64
+    seeded Drude clusters (P in 2..16, nmax in 4..7, one wavelength each in
+    400-900 nm, omega and 2 omega), all arriving at once.  Served by the
+    request loop with --slots concurrent slots, without grouping (every
+    request a one-system sweep in its slot) and with groups of up to --group
+    systems; beside them the plain API, one cluster after another
+    (omx_update + omx_solve) and in groups of similar padded order.  A solve
+    is one cluster at omega and 2 omega."""
+    from optimet_amd import solver
+    from optimet_amd.problem import drude_cluster
+    from optimet_amd.serving import Request, Server, summarize
+
+    rng = np.random.default_rng(5000)
+    cl = []
+    for i in range(64):
+        P, nmax = int(rng.integers(2, 17)), int(rng.integers(4, 8))
+        lam = float(rng.uniform(400e-9, 900e-9))
+        cl.append(drude_cluster(P, nmax, lam, 40, 80, 600 + 40 * P, 10, seed=5000 + i))
+    reqs = [Request(i, 0.0, c, np.array([c.wavelength])) for i, c in enumerate(cl)]
+    G = max(2, args.group)
+
+    def served(group):
+        srv = Server(slots=args.slots, kind="lu32ir", group=group)
+        try:
+            for _ in range(max(1, args.warmup)):  # tuning, first-touch allocations, member pools
+                srv.cache = type(srv.cache)(srv.cache.max_bytes)
+                srv.run(reqs, realtime=False)
+            srv.cache = type(srv.cache)(srv.cache.max_bytes)
+            res = srv.run(reqs, realtime=False)
+            groups = srv.groups_run
+        finally:
+            srv.close()
+        assert all(r.error is None for r in res), [r.error for r in res if r.error][:3]
+        summ = summarize(res, {r.id: 1 for r in reqs})
+        return summ, {r.id: r.out for r in res}, groups
+
+    order = sorted(range(len(cl)), key=lambda i: cl[i].N)
+
+    def api_one_by_one():
+        ctx = solver.Context()
+        t0 = time.perf_counter()
+        for i in order:
+            s = solver.Solver(cl[i], kind="lu32ir", context=ctx)
+            s.update()
+            s.solve()
+            s.cross_sections()
+        t = time.perf_counter() - t0
+        ctx.close()
+        return t
+
+    def api_grouped(pool, leader):
+        t0 = time.perf_counter()
+        for g0 in range(0, len(order), G):
+            mem = [solver.Solver(cl[i], kind="lu32ir", context=pool[k]) for k, i in enumerate(order[g0:g0 + G])]
+            for s in mem:
+                s.update()
+            solver.solve_group(mem, leader=leader)
+            for s in mem:
+                s.cross_sections()
+        return time.perf_counter() - t0
+
+    plain, plain_out, _ = served(0)
+    grouped, grouped_out, ngroups = served(G)
+    err = max(float(np.linalg.norm(grouped_out[i]["x_sca"] - plain_out[i]["x_sca"])
+                    / np.linalg.norm(plain_out[i]["x_sca"])) for i in plain_out)
+    cerr = max(float(np.max(np.abs(grouped_out[i]["xs"][:, 0] - plain_out[i]["xs"][:, 0])
+                            / np.abs(plain_out[i]["xs"][:, 0]))) for i in plain_out)
+    pool = [solver.Context() for _ in range(G)]
+    leader = solver.Context()
+    api_one_by_one()
+    api_grouped(pool, leader)
+    t1 = api_one_by_one()
+    t2 = api_grouped(pool, leader)
+    for c in pool + [leader]:
+        c.close()
+    line = {"metric": "cluster solves/sec, uneven single-wavelength requests batched across requests",
+            "value": grouped["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+            "vs_baseline": None, "dtype": "c64 LU (FP32-accurate bf16x3 MFMA products) + c128 GMRES-IR",
+            "data": "synthetic (seeded Drude clusters)",
+            "config": {"workload": "64 requests of one wavelength, P 2-16, nmax 4-7, N %d-%d, omega+2omega, %d slots, "
+                       "groups of up to %d systems" % (min(c.N for c in cl), max(c.N for c in cl), args.slots, G)},
+            "served_grouped": {k: grouped[k] for k in ("solves_per_s", "p50_s", "p99_s", "span_s")},
+            "served_plain": {k: plain[k] for k in ("solves_per_s", "p50_s", "p99_s", "span_s")},
+            "groups_run": ngroups, "speedup_vs_plain_serving": grouped["solves_per_s"] / plain["solves_per_s"],
+            "api_one_by_one_solves_per_s": len(cl) / t1, "api_grouped_solves_per_s": len(cl) / t2,
+            "max_rel_diff_x_sca": err, "max_rel_diff_cext": cerr}
+    print(json.dumps(line), flush=True)
+
+
 def config5_main(args):
     """Config 5: the seeded Poisson request stream (P ~ U{2..128}, nmax ~
     U{4..10}, 1-64 wavelengths, Drude or constant permittivity, omega and
@@ -325,7 +415,7 @@ def config5_main(args):
     nl = {r.id: r.wavelengths.size for r in reqs}
     out = {}
     for kind in (("auto",) if args.no_fp64 else ("auto", "lu64")):
-        srv = Server(slots=args.slots, kind=kind)
+        srv = Server(slots=args.slots, kind=kind, group=args.group)
 
         def progress(r, kind=kind):
             print(f"[{kind}] request {r.id} done at {r.finished:.1f} s (latency {r.latency:.1f} s"
@@ -339,6 +429,7 @@ def config5_main(args):
         summ = summarize(res, nl)
         summ["error_kinds"] = sorted({r.error.split("(")[0] for r in res if r.error})
         summ["cache"] = cache
+        summ["groups_run"] = srv.groups_run
         summ["latency_s"] = {r.id: r.latency for r in res if r.error is None}
         out[kind] = (summ, res)
     # roofline: the dominant kernel over a profiled replay of the first requests
@@ -377,7 +468,7 @@ def config5_main(args):
         "dtype": "c64 LU (FP32-accurate bf16x3 MFMA products) + c128 GMRES-IR; c128 GMRES over c64 GMRES cycles "
                  "from N = 16384", "data": "synthetic (seeded request stream)",
         "config": {"workload": "config5-serving-stream", "requests": args.requests, "slots": args.slots,
-                   "deadline_s": args.deadline, "rate_hz": 50.0, "solver": "auto"},
+                   "deadline_s": args.deadline, "rate_hz": 50.0, "solver": "auto", "group": args.group},
         "step": "the served sample; a solve = one wavelength of a request at omega and 2 omega",
         "p50_latency_s": m["p50_s"], "p99_latency_s": m["p99_s"], "mixed": m, "roofline": roof,
     }
@@ -478,7 +569,8 @@ def main():
                     help="config2: pivot search of the mixed path's factorisations")
     ap.add_argument("--legs", default="auto,gmres64,gmres32ir,lu32ir,lu64,aca",
                     help="config4: solver paths to run (a subset prints their figures only)")
-    ap.add_argument("--workload", choices=("config2", "config3", "config4", "config5"), default="config2",
+    ap.add_argument("--group", type=int, default=16, help="group: clusters per omx_solve_group")
+    ap.add_argument("--workload", choices=("config2", "config3", "config4", "config5", "group"), default="config2",
                     help="config2: one 64-sphere cluster (headline); config3: 256-wavelength batched sweep; "
                          "config4: 512 spheres, N = 49152, GMRES vs mixed LU; config5: request stream")
     ap.add_argument("--requests", type=int, default=8, help="config5: requests of the stream served")
@@ -516,6 +608,9 @@ def main():
         return
     if args.workload == "config5":
         config5_main(args)
+        return
+    if args.workload == "group":
+        group_main(args)
         return
 
     from optimet_amd.problem import config2

@@ -29,6 +29,16 @@ wavelength with the automatic solver choice (the mixed Krylov within the
 mixed LU's cost, else the mixed LU on the same matrices), one system at a
 time; smaller requests use the batched mixed-LU sweeps.
 
+With grouping on (`group` > 0), small requests (N <= GROUP_N, at most
+GROUP_LAMBDAS wavelengths, N within GROUP_RATIO of the oldest one's) that
+have arrived are gathered, oldest first, into
+one job of at most `group` systems (one per request and wavelength): each
+system is assembled in a member context of the slot's pool, and the members
+are solved together by `omx_solve_group` (batched LU and lockstep refinement
+over the identity-bordered systems, in the slot's context). Systems of
+different requests, clusters and sizes thus share one batched launch. The
+job's footprint is the members' own buffers plus the batch.
+
 Results are cached by content: a wavelength of a cluster (geometry,
 materials, excitation) that an earlier request solved is answered from the
 cache without device work, and the server counts hits and misses.
@@ -47,6 +57,9 @@ from . import solver as _solver
 
 
 KRYLOV_N = 16384  # "auto": requests from this order on are solved one wavelength at a time
+GROUP_N = 4096     # with grouping on: requests up to this order ...
+GROUP_LAMBDAS = 8  # ... and this many wavelengths may be solved in one group,
+GROUP_RATIO = 1.5  # with orders within this factor of the oldest one's (padding costs flops)
 
 
 def footprint(cluster, batch, kind):
@@ -146,8 +159,13 @@ class Server:
     start); batch_cap: most wavelengths per batched sweep."""
 
     def __init__(self, slots=3, budget=None, kind="lu32ir", device=0, batch_cap=32, solve_fn=None,
-                 cache_bytes=4 << 30, profile=False):
+                 cache_bytes=4 << 30, profile=False, group=0, group_fn=None):
         self.kind = kind
+        self.group = 0 if profile else int(group)  # most systems per group job (0: no grouping)
+        self._group_fn = group_fn  # stand-in for a group job (CPU tests)
+        self._pools = {}  # slot -> member contexts of its group jobs
+        self.device = device
+        self.groups_run = 0
         self.cache = ResultCache(cache_bytes)
         self.profile = profile  # per-launch HIP events; each request's counters kept in prof_log
         self.prof_log = []
@@ -169,7 +187,67 @@ class Server:
     def close(self):
         for c in self._contexts:
             c.close()
+        for pool in self._pools.values():
+            for c in pool:
+                c.close()
         self._contexts = []
+        self._pools = {}
+
+    def groupable(self, req):
+        return (self.group > 0 and self.kind in ("lu32ir", "lu64", "auto") and req.cluster.N <= GROUP_N
+                and req.wavelengths.size <= GROUP_LAMBDAS)
+
+    def group_footprint(self, reqs):
+        """Members' own buffers (one system each) plus the batch at the
+        largest member's padded order."""
+        kind = "lu64" if self.kind == "lu64" else "lu32ir"
+        members = [r.cluster for r in reqs for _ in range(r.wavelengths.size)]
+        big = max(members, key=lambda c: c.N)
+        return sum(footprint(c, 1, kind) for c in members) + footprint(big, len(members), kind)
+
+    def _solve_group(self, slot, reqs):
+        """One group job: every (request, wavelength) not in the cache as a
+        member system, solved together; returns each request's result in the
+        sweep's layout."""
+        if self._group_fn is not None:
+            return self._group_fn(slot, reqs)
+        keys = {r.id: cluster_key(r.cluster) for r in reqs}
+        cached = {(r.id, float(l)): self.cache.get((keys[r.id], float(l))) for r in reqs for l in r.wavelengths}
+        todo = [(r, float(l)) for r in reqs for l in r.wavelengths if cached[(r.id, float(l))] is None]
+        fresh = {}
+        if todo:
+            pool = self._pools.setdefault(slot, [])
+            while len(pool) < len(todo):
+                pool.append(_solver.Context(self.device))
+            kind = "lu64" if self.kind == "lu64" else "lu32ir"
+            leader = self._contexts[slot]
+            try:
+                mem = []
+                for k, (r, lam) in enumerate(todo):
+                    sv = _solver.Solver(r.cluster.with_wavelength(lam), kind=kind, context=pool[k])
+                    sv.update()
+                    mem.append(sv)
+                outs = _solver.solve_group(mem, leader=leader)
+                for (r, lam), sv, o in zip(todo, mem, outs):
+                    cs = sv.cross_sections()
+                    st = sv.stats()
+                    v = {"x_sca": o["x_sca"],
+                         "xs": np.array([cs["cext"], cs["csca"], cs["cabs"], cs["csca_sh"], cs["cabs_sh"]]),
+                         "iterations": np.array([st["iterations"], st["iterations_sh"]], dtype=np.int32)}
+                    if r.cluster.sh:
+                        v["x_sca_sh"] = o["x_sca_sh"]
+                    fresh[(r.id, lam)] = v
+                    self.cache.put((keys[r.id], lam), v)
+            finally:
+                for c in pool[:len(todo)]:
+                    c.trim()
+                leader.trim()
+            self.groups_run += 1
+        out = {}
+        for r in reqs:
+            per = [fresh.get((r.id, float(l))) or cached[(r.id, float(l))] for l in r.wavelengths]
+            out[r.id] = {k: np.stack([p[k] for p in per]) for k in per[0]}
+        return out
 
     def plan(self, req):
         """Sweep batch size and footprint of a request: the largest batch
@@ -227,6 +305,24 @@ class Server:
                 self.prof_log.append((req.id, ctx.stats()))
             ctx.trim()
 
+    def _group_worker(self, slot, reqs, ress, nbytes):
+        try:
+            outs = self._solve_group(slot, reqs)
+            for r, res in zip(reqs, ress):
+                res.out = outs[r.id]
+        except Exception as e:  # reported per request; the server keeps going
+            for res in ress:
+                res.error = repr(e)
+        t = time.perf_counter() - self._t0
+        for res in ress:
+            res.finished = t
+            if self._on_done is not None:
+                self._on_done(res)
+        with self._lock:
+            self._held -= nbytes
+            self._free_slots.append(slot)
+            self._lock.notify_all()
+
     def _worker(self, slot, req, res):
         try:
             res.out = self._solve(slot, req, res.batch)
@@ -264,6 +360,38 @@ class Server:
                 if realtime and req.arrival > now:
                     self._lock.wait(timeout=req.arrival - now)
                     continue
+                if self.groupable(req):
+                    # the head and the small requests that have arrived after
+                    # it, oldest first, up to `group` systems and the budget
+                    grp, nsys = [req], req.wavelengths.size
+                    for cand in list(pending)[1:]:
+                        if realtime and cand.arrival > now:
+                            break
+                        if (self.groupable(cand) and nsys + cand.wavelengths.size <= self.group
+                                and max(cand.cluster.N, req.cluster.N) <= GROUP_RATIO * min(cand.cluster.N,
+                                                                                           req.cluster.N)
+                                and self.group_footprint(grp + [cand]) <= self.budget):
+                            grp.append(cand)
+                            nsys += cand.wavelengths.size
+                    if len(grp) > 1:
+                        gbytes = self.group_footprint(grp)
+                        if gbytes <= self.budget:
+                            if not self._free_slots or self._held + gbytes > self.budget:
+                                self._lock.wait(timeout=0.05)
+                                continue
+                            for r in grp:
+                                pending.remove(r)
+                            slot = self._free_slots.pop()
+                            self._held += gbytes
+                            ress = [results[r.id] for r in grp]
+                            for res in ress:
+                                res.admitted, res.batch, res.bytes = now, nsys, 0
+                            ress[0].bytes = gbytes
+                            th = threading.Thread(target=self._group_worker, args=(slot, grp, ress, gbytes),
+                                                  daemon=True)
+                            threads.append(th)
+                            th.start()
+                            continue
                 batch, nbytes = self.plan(req)
                 if nbytes > self.budget:
                     res = results[req.id]

@@ -176,3 +176,73 @@ def test_result_cache_answers_repeated_wavelengths():
     c.cluster.eps_r[0] += 1e-12
     s.run([Request(2, 0.0, c.cluster, a.wavelengths)], realtime=False)
     assert seen[-1] == list(a.wavelengths)
+
+
+def test_small_requests_are_grouped_oldest_first():
+    """With grouping on, small requests that have arrived go into one job of
+    at most `group` systems (one per request and wavelength), oldest first; a
+    request above the size limit and one not yet arrived are not taken in."""
+    reqs = [_req(i, 0.0, nl=2) for i in range(5)] + [_req(5, 0.0, P=40, nmax=8, nl=1), _req(6, 0.3, nl=1)]
+    s, fake = _server(slots=2, budget=1 << 50)
+    s.group = 6
+    jobs = []
+
+    def group_fn(slot, grp):
+        jobs.append([r.id for r in grp])
+        time.sleep(0.05)
+        return {r.id: {"group": len(grp)} for r in grp}
+
+    s._group_fn = group_fn
+    res = s.run(reqs, realtime=True)
+    assert all(r.error is None for r in res), [r.error for r in res]
+    assert jobs[0] == [0, 1, 2]  # 6 systems: three requests of two wavelengths
+    assert sorted(i for j in jobs for i in j) == [0, 1, 2, 3, 4]
+    assert all(len(j) >= 2 for j in jobs)
+    assert fake.order == [5, 6]  # the large one and the late single one run alone
+    assert res[3].out["group"] == 2 and res[0].admitted <= res[3].admitted
+    assert s.group_footprint([reqs[0], reqs[1]]) >= footprint(reqs[0].cluster, 4, "lu32ir")
+
+
+def test_group_respects_the_budget():
+    reqs = [_req(i, 0.0, nl=1) for i in range(6)]
+    s, fake = _server(slots=3, budget=1 << 50)
+    s.group = 8
+    s.budget = s.group_footprint(reqs[:3])  # room for three systems per job
+    sizes = []
+    s._group_fn = lambda slot, grp: (sizes.append(len(grp)), {r.id: {} for r in grp})[1]
+    res = s.run(reqs, realtime=False)
+    assert all(r.error is None for r in res)
+    assert max(sizes) <= 3 and sum(sizes) + len(fake.order) == 6
+
+
+@pytest.mark.gpu
+def test_grouped_requests_match_their_sweeps():
+    """Small requests served in one group job (omx_solve_group over every
+    request and wavelength) give the coefficients and cross sections of the
+    same requests served one at a time by batched sweeps, to 1e-10."""
+    reqs = [_req(0, 0.0, P=4, nmax=4, nl=2), _req(1, 0.0, P=6, nmax=3, nl=1), _req(2, 0.0, P=3, nmax=5, nl=3),
+            _req(3, 0.0, P=2, nmax=6, nl=2)]
+    print("grouped requests: N =", [r.cluster.N for r in reqs], flush=True)
+    alone = {}
+    srv = Server(slots=1, budget=8 << 30, kind="lu32ir")
+    try:
+        for r in reqs:
+            (res,) = srv.run([r], realtime=False)
+            assert res.error is None, res.error
+            alone[r.id] = res.out
+    finally:
+        srv.close()
+    srv = Server(slots=2, budget=8 << 30, kind="lu32ir", group=16)
+    try:
+        grouped = srv.run(reqs, realtime=False)
+        assert srv.groups_run == 1
+    finally:
+        srv.close()
+    for res in grouped:
+        assert res.error is None, res.error
+        assert res.batch == 8  # every request's wavelengths in one job
+        a = alone[res.id]
+        for key in ("x_sca", "x_sca_sh"):
+            d = np.linalg.norm(res.out[key] - a[key]) / np.linalg.norm(a[key])
+            assert d < 1e-10, (res.id, key, d)
+        assert np.allclose(res.out["xs"], a["xs"], rtol=1e-10, atol=0)
