@@ -983,18 +983,21 @@ __device__ inline void split_bf16x3(const float *f, u32x4 &h, u32x4 &m, u32x4 &l
   }
 }
 
-template <int TM, int TN, int KD, int WGM, int WGN, int BK, int GM>
+// DB: the split planes double-buffered in LDS, so that each wave splits and
+// stores chunk c + 1 while its products of chunk c run (one barrier per
+// chunk); the products and their order are those of the single buffer.
+template <int TM, int TN, int KD, int WGM, int WGN, int BK, int GM, bool DB = false>
 __global__ __launch_bounds__(64 * WGM * WGN) void gemm_split(float2 *A, size_t lda, int row0, int col0, int k0, int M,
                                                             int N, size_t sA) {
   A += blockIdx.z * sA;
   constexpr int NT = 64 * WGM * WGN;
   constexpr int WM = TM / WGM, WN = TN / WGN, FM = WM / 32, FN = WN / 32;
-  constexpr int KG = BK / 4, NC = KD / BK;
+  constexpr int KG = BK / 4, NC = KD / BK, NBUF = DB ? 2 : 1;
   constexpr int UA = TM * KG / NT, UB = TN * KG / NT;
   static_assert(UA * NT == TM * KG && UB * NT == TN * KG, "tile not divisible");
   static_assert(FM >= 1 && FN >= 1 && BK % 8 == 0 && KD % BK == 0, "tile shape");
-  __shared__ u32x4 Ys[3][KG][TM];
-  __shared__ u32x4 Xs[3][KG][TN];
+  __shared__ u32x4 Ys[NBUF][3][KG][TM];
+  __shared__ u32x4 Xs[NBUF][3][KG][TN];
   const int gx = (M + TM - 1) / TM, gy = (N + TN - 1) / TN, T = gx * gy;
   int L = blockIdx.x;
   const int full = T / 8 * 8;
@@ -1069,32 +1072,48 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_split(float2 *A, size_t l
         }
       }
   };
-  gload(0, 0);
-  if(NC > 1) gload(BK, 1);
-#pragma unroll
-  for(int c = 0; c < NC; ++c) {
-    const int sl = c & 1;
-    __syncthreads();  // the previous chunk has been read out of LDS
+  // chunk c's operands from register slot sl, split into LDS buffer buf
+  auto split_store = [&](int sl, int buf) {
 #pragma unroll
     for(int i = 0; i < UA; ++i) {
       const int u = t + NT * i, m = u % TM, kg = u / TM;
       u32x4 h, md, lo;
       split_bf16x3(ga[sl][i], h, md, lo);
-      Ys[0][kg][m] = h;
-      Ys[1][kg][m] = md;
-      Ys[2][kg][m] = lo;
+      Ys[buf][0][kg][m] = h;
+      Ys[buf][1][kg][m] = md;
+      Ys[buf][2][kg][m] = lo;
     }
 #pragma unroll
     for(int i = 0; i < UB; ++i) {
       const int u = t + NT * i, kg = u % KG, n = u / KG;
       u32x4 h, md, lo;
       split_bf16x3(gb[sl][i], h, md, lo);
-      Xs[0][kg][n] = h;
-      Xs[1][kg][n] = md;
-      Xs[2][kg][n] = lo;
+      Xs[buf][0][kg][n] = h;
+      Xs[buf][1][kg][n] = md;
+      Xs[buf][2][kg][n] = lo;
     }
+  };
+  gload(0, 0);
+  if(NC > 1) gload(BK, 1);
+  if constexpr(DB) {
+    split_store(0, 0);
     __syncthreads();
-    if(c + 2 < NC) gload((c + 2) * BK, sl);
+    if(NC > 2) gload(2 * BK, 0);
+  }
+#pragma unroll
+  for(int c = 0; c < NC; ++c) {
+    const int sl = c & 1, buf = DB ? (c & 1) : 0;
+    if constexpr(DB) {
+      // chunk c + 1 into the other buffer (read by every wave before the
+      // barrier that closed the previous chunk), then its slot refilled
+      if(c + 1 < NC) split_store((c + 1) & 1, buf ^ 1);
+      if(c + 3 < NC) gload((c + 3) * BK, (c + 1) & 1);
+    } else {
+      __syncthreads();  // the previous chunk has been read out of LDS
+      split_store(sl, 0);
+      __syncthreads();
+      if(c + 2 < NC) gload((c + 2) * BK, sl);
+    }
     if(c + 1 == NC) cload();
 #pragma unroll
     for(int ks = 0; ks < BK / 8; ++ks) {
@@ -1104,10 +1123,10 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_split(float2 *A, size_t l
 #pragma unroll
       for(int s = 0; s < 3; ++s) {
 #pragma unroll
-        for(int b = 0; b < FN; ++b) x[s][b] = Xs[s][kg][wn * WN + 32 * b + li];
+        for(int b = 0; b < FN; ++b) x[s][b] = Xs[buf][s][kg][wn * WN + 32 * b + li];
 #pragma unroll
         for(int a = 0; a < FM; ++a) {
-          const u32x4 y = Ys[s][kg][wm * WM + 32 * a + li];
+          const u32x4 y = Ys[buf][s][kg][wm * WM + 32 * a + li];
 #pragma unroll
           for(int d = 0; d < 4; ++d) {
             yr[s][a][d] = y[d] ^ 0x80000000u;
@@ -1127,6 +1146,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_split(float2 *A, size_t l
             accI[a][b] = mfma_bf16_32(x[SX[p]][b], yi[SY[p]][a], accI[a][b]);
           }
     }
+    if constexpr(DB) __syncthreads();  // chunk c read out; chunk c + 1 stored
   }
 #pragma unroll
   for(int a = 0; a < FM; ++a)
@@ -1150,6 +1170,9 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_split(float2 *A, size_t l
 // random data).
 struct SplitTile {
   static constexpr int TM = 128, TN = 128, WGM = 4, WGN = 2, BK = 16, GM = 8;
+  // double-buffered split planes: bit-identical, 149 against 125 TFLOP/s at
+  // M = N = 10112 (one matrix), 156 against 154 for 64 systems of 3712
+  static constexpr bool DB = true;
 };
 
 // The bulk update with K = 2 NB: split-bf16 for FP32 factors when `split`,
@@ -1168,7 +1191,7 @@ void bulk_update(T *A, size_t lda, int row0, int col0, int k0, int M, int N, hip
     if(split) {
       using U = SplitTile;
       const int tiles = ((M + U::TM - 1) / U::TM) * ((N + U::TN - 1) / U::TN);
-      hipLaunchKernelGGL((gemm_split<U::TM, U::TN, 2 * kNB, U::WGM, U::WGN, U::BK, U::GM>), dim3(tiles, 1, nb),
+      hipLaunchKernelGGL((gemm_split<U::TM, U::TN, 2 * kNB, U::WGM, U::WGN, U::BK, U::GM, U::DB>), dim3(tiles, 1, nb),
                          dim3(64 * U::WGM * U::WGN), 0, st, A, lda, row0, col0, k0, M, N, sA);
       return;
     }
