@@ -186,7 +186,11 @@ def sweep_main(args):
     c = config3_cluster()
     lams = config3_wavelengths(256)
     graphs = args.graphs == "on"
-    mixed = run_sweep(c, lams, "lu32ir", args.steps, args.warmup, args.batch, graphs=graphs)
+    # the value: the automatic choice (S by lockstep FP64 GMRES beside V's
+    # batched mixed LU); beside it the mixed LU for both and the FP64 LU
+    mixed = run_sweep(c, lams, args.sweep_kind, args.steps, args.warmup, args.batch, graphs=graphs)
+    mixed_lu = run_sweep(c, lams, "lu32ir", args.steps, args.warmup, args.batch, graphs=graphs) \
+        if args.sweep_kind != "lu32ir" else None
     fp64 = None if args.no_fp64 else run_sweep(c, lams, "lu64", args.steps, args.warmup, args.batch, graphs=graphs)
     nl = lams.size
     value = args.steps * nl / mixed["total"]
@@ -195,9 +199,12 @@ def sweep_main(args):
         "metric": "cluster solves/sec + p99 latency, mixed-precision vs FP64 LU at rel err <=1e-9",
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * mixed["total"] / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "c64 LU (FP32-accurate bf16x3 MFMA products) + c128 GMRES-IR", "data": "synthetic (seeded Drude cluster)",
+        "vs_baseline": None,
+        "dtype": ("c128 GMRES (omega, lockstep) beside c64 LU (FP32-accurate bf16x3 MFMA products) + c128 GMRES-IR "
+                  "(2 omega)") if args.sweep_kind == "auto" else "c64 LU (FP32-accurate bf16x3 MFMA products) + c128 GMRES-IR",
+        "data": "synthetic (seeded Drude cluster)",
         "config": {"workload": c.meta["name"], "P": c.nobj, "nmax": c.nmax, "N": c.N, "wavelengths": nl,
-                   "frequencies": "omega+2omega", "solver": "lu32ir", "batch": ms["sweep_batch"],
+                   "frequencies": "omega+2omega", "solver": args.sweep_kind, "batch": ms["sweep_batch"],
                    "graphs": args.graphs},
         "step": "one sweep of all wavelengths (a solve = one wavelength at omega and 2 omega)",
         "p50_ms_per_sweep": 1e3 * pct(mixed["lat"], 50), "p99_ms_per_sweep": 1e3 * pct(mixed["lat"], 99),
@@ -205,7 +212,14 @@ def sweep_main(args):
         "roofline": matvec_roofline(mixed["prof"]),
         "autotune": {"lu32ir": tuned(ms["Np"], ms["sweep_batch"], False),
                      "lu64": tuned(ms["Np"], ms["sweep_batch"], True) if fp64 is not None else None},
+        "systems_redone_alone": ms["pivot_fallbacks"],
     }
+    if mixed_lu is not None:
+        ls = mixed_lu["stats"]
+        line["mixed_lu"] = {"solves_per_s": args.steps * nl / mixed_lu["total"],
+                            "p50_ms_per_sweep": 1e3 * pct(mixed_lu["lat"], 50),
+                            "max_iterations": [ls["iterations"], ls["iterations_sh"]],
+                            "roofline": matvec_roofline(mixed_lu["prof"])}
     if fp64 is not None:
         fs = fp64["stats"]
         rel = lambda a, b: float(np.max(np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)))
@@ -213,6 +227,8 @@ def sweep_main(args):
                            "p50_ms_per_sweep": 1e3 * pct(fp64["lat"], 50),
                            "max_iterations": [fs["iterations"], fs["iterations_sh"]]}
         line["speedup_vs_fp64"] = value / line["fp64_lu"]["solves_per_s"]
+        if mixed_lu is not None:
+            line["mixed_lu"]["speedup_vs_fp64"] = line["mixed_lu"]["solves_per_s"] / line["fp64_lu"]["solves_per_s"]
         mo, fo = mixed["out"], fp64["out"]
         line["max_rel_err_vs_fp64"] = {"x_sca": rel(mo["x_sca"], fo["x_sca"]),
                                        "x_sca_sh": rel(mo["x_sca_sh"], fo["x_sca_sh"]),
@@ -570,6 +586,8 @@ def main():
     ap.add_argument("--legs", default="auto,gmres64,gmres32ir,lu32ir,lu64,aca",
                     help="config4: solver paths to run (a subset prints their figures only)")
     ap.add_argument("--group", type=int, default=16, help="group: clusters per omx_solve_group")
+    ap.add_argument("--sweep-kind", choices=("auto", "lu32ir"), default="auto",
+                    help="config3: solver kind of the mixed sweep")
     ap.add_argument("--workload", choices=("config2", "config3", "config4", "config5", "group"), default="config2",
                     help="config2: one 64-sphere cluster (headline); config3: 256-wavelength batched sweep; "
                          "config4: 512 spheres, N = 49152, GMRES vs mixed LU; config5: request stream")

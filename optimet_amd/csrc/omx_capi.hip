@@ -91,6 +91,10 @@ inline int auto_krylov_steps(int N) { return std::max(4, (int)(0.0143 * N / (kIn
 // 16 N^2-byte FP64 product at about 6 TB/s: N / 125); S falls back to its own
 // mixed LU when GMRES has not converged by then.
 inline int auto_gmres64_steps(int N) { return std::max(20, N / 125); }
+// Batched sweeps take the same rule (S by lockstep FP64 GMRES beside V's
+// batched factorisation, within twice that budget) from this order on; below
+// it the budget (2 N / 125 < 33 products) is under what such systems need.
+constexpr int kSweepKrylovN = 2048;
 
 const double kMu0 = 4.0 * omx::kPi * 1e-7;
 const double kEps0 = 1.0 / (kMu0 * omx::kC0 * omx::kC0);
@@ -1305,6 +1309,40 @@ struct GmresB {
 };
 
 // Buffers of a batch of nb systems of order Np (mixed: FP32 copies too).
+// Enqueues V's factorisation from a host thread of its own: the enqueueing
+// of a factorisation proceeds at the device's pace, so enqueued from the
+// calling thread it would hold back S's iterative solve (which needs the host
+// between its products) until V's factorisation had run.  join() rethrows an
+// error of the thread; the destructor joins without rethrowing.
+struct SideEnqueue {
+  std::thread t;
+  std::exception_ptr err;
+  void start(omx_context *c, void (*f)(void *)) {
+    run(c->device, [c, f] { f(c); });
+  }
+  void run(int device, std::function<void()> job) {
+    t = std::thread([this, device, job] {
+      try {
+        HIPCHK(hipSetDevice(device));
+        job();
+      } catch(...) {
+        err = std::current_exception();
+      }
+    });
+  }
+  void join() {
+    if(t.joinable()) t.join();
+    if(err) {
+      std::exception_ptr e = err;
+      err = nullptr;
+      std::rethrow_exception(e);
+    }
+  }
+  ~SideEnqueue() {
+    if(t.joinable()) t.join();
+  }
+};
+
 void sweep_alloc(omx_context *c, int nb, bool f64, bool sh) {
   auto &w = c->sw;
   size_t *T = &c->total;
@@ -1554,7 +1592,7 @@ TuneChoice single_tune(omx_context *c, bool f64) {
 // one at a time with partial pivoting.
 void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, const cplx *eps_sh, double *x_sca,
                  double *x_int, double *x_sca_sh, double *x_int_sh, double *xs, std::vector<int> &bad,
-                 std::vector<int> &iters, std::vector<double> &resid) {
+                 std::vector<int> &iters, std::vector<double> &resid, bool krylov_s) {
   const bool f64 = c->kind == OMX_SOLVER_LU64, sh = c->sh;
   sweep_alloc(c, nb, f64, sh);
   const TuneChoice tune = c->autotune ? sweep_tune(c, nb, f64) : TuneChoice{c->lu32_gemm == OMX_GEMM_BF16X3, 0};
@@ -1667,7 +1705,17 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
     lwV.trsv = tune.trsv;
   }
   void *FS = f64 ? w.F64.p : w.S32.p, *FV = sh ? (f64 ? w.F64sh.p : w.V32.p) : nullptr;
-  if(c->graphs) {
+  SideEnqueue vq;
+  if(krylov_s) {
+    // S is solved by GMRES (no factorisation); V's factorisation is enqueued
+    // from a host thread of its own so that S's iterations start at once
+    HIPCHK(hipEventRecord(c->xev[0], st));
+    vq.run(c->device, [&] {
+      HIPCHK(hipStreamWaitEvent(c->st2, c->xev[0], 0));
+      factor(true, lwV, c->st2);
+      HIPCHK(hipEventRecord(c->xev[1], c->st2));
+    });
+  } else if(c->graphs) {
     // captured once per batch shape and buffers, then replayed chunk after chunk
     auto &g = c->fg_sweep;
     std::vector<uintptr_t> key{2, (uintptr_t)nb, (uintptr_t)Np, (uintptr_t)f64, (uintptr_t)sh, (uintptr_t)tune.split};
@@ -1719,7 +1767,7 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
     factor(false, lwS, st);
   }
   // refinement of every system of S (V); per-system convergence flags
-  auto refine = [&](bool second, omx::LuWork &lw, void *F, const cplx *rhs, cplx *x) {
+  auto refine = [&](bool second, omx::LuWork &lw, void *F, const cplx *rhs, cplx *x, bool from_x = false) {
     status_scan(bad, second);
     cplx *A64 = second ? cV64 : cS64;
     omx::LuWork *lwp = &lw;
@@ -1733,7 +1781,7 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
       gm.prec = [c, F, lwp, Np, lda](const cplx *in, cplx *out) {
         omx::lu_solve<float2>(static_cast<const float2 *>(F), Np, lda, *lwp, in, out, c->st);
       };
-    gm.prec(rhs, x);
+    if(!from_x) gm.prec(rhs, x);  // else: start from the iterate x holds
     std::vector<double> res;
     std::vector<int> its;
     // the batch runs in lockstep until its slowest system stops: a system
@@ -1748,7 +1796,35 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
       resid[2 * z + (second ? 1 : 0)] = res[z];
     }
   };
-  refine(false, lwS, FS, w.Q.as<cplx>(), w.x.as<cplx>());
+  if(krylov_s) {
+    // lockstep FP64 GMRES on every system of S, from zero, within twice the
+    // single-system budget (S's solve runs beside V's factorisation); a
+    // system short of the target is left to the one-at-a-time path
+    status_scan(bad, false);
+    GmresB gm{c, cS64, n2, Np, nb, nullptr, w};
+    cplx *Sx = w.x.as<cplx>();
+    gm.prec = [c, Np, nb](const cplx *in, cplx *out) { omx::vcopy(out, in, Np * nb, c->st); };
+    HIPCHK(hipMemsetAsync(Sx, 0, sizeof(cplx) * lda * nb, st));
+    std::vector<double> res;
+    std::vector<int> its;
+    const int budget = std::min(c->maxit, c->auto_budget > 0 ? c->auto_budget : 2 * auto_gmres64_steps(c->N));
+    gm.solve(w.Q.as<cplx>(), Sx, c->tol, budget, c->restart, res, its);
+    bool short_any = false;
+    for(int z = 0; z < nb; ++z) {
+      short_any |= !(res[z] <= 1e3 * c->tol);
+      iters[2 * z] = its[z];
+      resid[2 * z] = res[z];
+    }
+    if(short_any) {
+      // S factored for the batch after all; every system refined from its
+      // GMRES iterate (those already at the target stop at once)
+      factor(false, lwS, st);
+      refine(false, lwS, FS, w.Q.as<cplx>(), Sx, true);
+      ++c->fallbacks;
+    }
+  } else {
+    refine(false, lwS, FS, w.Q.as<cplx>(), w.x.as<cplx>());
+  }
   for(int z = 0; z < nb; ++z)
     omx::vmul_ov(w.xint.as<cplx>() + z * lda, w.x.as<cplx>() + z * lda, w.ov.as<cplx>() + z * ovs, omx::OV_IAUX, P,
                  nmax, nullptr, N, st);
@@ -1768,6 +1844,7 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
                      w.xint.as<cplx>() + z * lda, w.K.as<cplx>() + z * lda, w.K1.as<cplx>() + z * lda};
       omx::run_shsrc(sa, st);
     }
+    vq.join();  // V's factorisation enqueued (and its event recorded) before the wait below
     HIPCHK(hipStreamWaitEvent(st, c->xev[1], 0));  // V's factorisation
     refine(true, lwV, FV, w.K.as<cplx>(), w.xsh.as<cplx>());
     for(int z = 0; z < nb; ++z)
@@ -2236,37 +2313,6 @@ void sh_source(omx_context *c, hipStream_t st) {
                  c->d_xint.as<cplx>(), c->d_K.as<cplx>(), c->d_K1.as<cplx>()};
   omx::run_shsrc(sa, st);
 }
-
-// Enqueues V's factorisation from a host thread of its own: the enqueueing
-// of a factorisation proceeds at the device's pace, so enqueued from the
-// calling thread it would hold back S's iterative solve (which needs the host
-// between its products) until V's factorisation had run.  join() rethrows an
-// error of the thread; the destructor joins without rethrowing.
-struct SideEnqueue {
-  std::thread t;
-  std::exception_ptr err;
-  void start(omx_context *c, void (*f)(void *)) {
-    t = std::thread([this, c, f] {
-      try {
-        HIPCHK(hipSetDevice(c->device));
-        f(c);
-      } catch(...) {
-        err = std::current_exception();
-      }
-    });
-  }
-  void join() {
-    if(t.joinable()) t.join();
-    if(err) {
-      std::exception_ptr e = err;
-      err = nullptr;
-      std::rethrow_exception(e);
-    }
-  }
-  ~SideEnqueue() {
-    if(t.joinable()) t.join();
-  }
-};
 
 int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, double *x_int_sh) {
   return guard([&] {
@@ -2845,7 +2891,10 @@ int omx_solve_sweep(omx_context *c, int nl, const double *lambdas, const double 
   return guard([&] {
     if(!c || !c->have_geom || !c->have_exc || nl < 1 || !lambdas || batch < 0)
       throw OmxError(OMX_ERR_ARG, "invalid sweep (geometry, excitation and wavelengths required)");
-    if(c->kind_req == OMX_SOLVER_AUTO) c->kind = OMX_SOLVER_LU32_IR;  // batched sweeps factor
+    // the automatic choice in a batched sweep: V's mixed LU, and with the
+    // second harmonic S by lockstep FP64 GMRES beside V's factorisation
+    if(c->kind_req == OMX_SOLVER_AUTO) c->kind = OMX_SOLVER_LU32_IR;
+    const bool krylov_s = c->kind_req == OMX_SOLVER_AUTO && c->sh && c->N >= kSweepKrylovN;
     if(!is_lu(c->kind)) throw OmxError(OMX_ERR_ARG, "sweeps need an LU solver kind");
     for(int l = 0; l < nl; ++l)
       if(!(lambdas[l] > 0.0)) throw OmxError(OMX_ERR_ARG, "non-positive wavelength");
@@ -2884,7 +2933,7 @@ int omx_solve_sweep(omx_context *c, int nl, const double *lambdas, const double 
       std::vector<double> res(2 * n, 0.0);
       sweep_chunk(c, n, lambdas + l0, eps.data() + (size_t)l0 * P, eps_sh.data() + (size_t)l0 * P,
                   off(x_sca, l0, N), off(x_int, l0, N), off(x_sca_sh, l0, N), off(x_int_sh, l0, N),
-                  xs ? xs + 5 * l0 : nullptr, bad, its, res);
+                  xs ? xs + 5 * l0 : nullptr, bad, its, res, krylov_s);
       for(int z = 0; z < n; ++z) {
         if(bad[z]) redo.push_back(l0 + z);
         if(iterations) {

@@ -27,7 +27,9 @@ is at most the batch's.
 With kind "auto" a request of N >= 16384 unknowns is solved wavelength by
 wavelength with the automatic solver choice (the mixed Krylov within the
 mixed LU's cost, else the mixed LU on the same matrices), one system at a
-time; smaller requests use the batched mixed-LU sweeps.
+time; smaller requests use batched sweeps with the automatic choice (V's
+batched mixed LU and, from N = 2048 with the second harmonic, S by lockstep
+FP64 GMRES beside it).
 
 With grouping on (`group` > 0), small requests (N <= GROUP_N, at most
 GROUP_LAMBDAS wavelengths, N within GROUP_RATIO of the oldest one's) that
@@ -297,8 +299,7 @@ class Server:
                     outs["xs"].append([cs["cext"], cs["csca"], cs["cabs"], cs["csca_sh"], cs["cabs_sh"]])
                     outs["iterations"].append([st["iterations"], st["iterations_sh"]])
                 return {k: np.array(v) for k, v in outs.items() if len(v)}
-            kind = "lu32ir" if self.kind == "auto" else self.kind
-            s = _solver.Solver(c, kind=kind, context=ctx)
+            s = _solver.Solver(c, kind=self.kind, context=ctx)
             return s.sweep(lams, batch=batch, outputs=("x_sca", "x_sca_sh"))
         finally:
             if self.profile:

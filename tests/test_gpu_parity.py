@@ -326,12 +326,13 @@ def test_golden_fixtures_gpu(ctx, kind, tol):
             assert abs(cs["csca_sh"] - ref["csca_sh"]) <= tol * abs(ref["csca_sh"]), name
 
 
-@pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL)])
+@pytest.mark.parametrize("kind,tol", [("lu64", FP64_TOL), ("lu32ir", MIXED_TOL), ("auto", MIXED_TOL)])
 @pytest.mark.parametrize("P,nmax,batch", [(5, 4, 3), (8, 6, 4)])
 def test_wavelength_sweep_batched(ctx, ora, kind, tol, P, nmax, batch):
     """omx_solve_sweep: several wavelengths solved as batches (chunks of
     `batch` systems, the last one partial) against the oracle at each
-    wavelength, Drude permittivities re-evaluated per wavelength."""
+    wavelength, Drude permittivities re-evaluated per wavelength.  "auto"
+    solves S by lockstep FP64 GMRES beside V's batched mixed LU."""
     from optimet_amd import solver
     c = drude_cluster(P, nmax, 600e-9, 40, 80, 600 if P < 8 else 800, 10, seed=31 + P)
     lams = np.linspace(450e-9, 850e-9, 7)

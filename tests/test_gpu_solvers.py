@@ -116,3 +116,58 @@ def test_auto_small_sh_system_falls_back_to_the_lu():
     assert st_a["iterations"] == st_l["iterations"] and st_a["iterations_sh"] == st_l["iterations_sh"]
     for k in ("x_sca", "x_sca_sh"):
         assert rel(out_a[k], out_l[k]) < 1e-13, k
+
+
+@pytest.fixture(scope="module")
+def ora():
+    import oracle
+    return oracle
+
+
+@pytest.fixture(scope="module")
+def sweep_case():
+    """14 Drude spheres at nmax 8 (N = 2240), eight wavelengths, with SH: the
+    automatic sweep solves S by lockstep FP64 GMRES beside V's batched LU."""
+    c = drude_cluster(14, 8, 700e-9, 40, 80, 1000, 10, seed=91)
+    return c, np.linspace(450e-9, 850e-9, 8)
+
+
+def sweep(c, lams, kind, budget=0):
+    from optimet_amd import solver
+    s = solver.Solver(c, kind=kind)
+    s.ctx.set_auto_budget(budget)
+    out = s.sweep(lams, batch=4)
+    st = s.stats()
+    s.ctx.close()
+    return out, st
+
+
+def test_auto_sweep_gmres_for_s(ora, sweep_case):
+    c, lams = sweep_case
+    print(f"auto sweep: N {c.N}, {lams.size} wavelengths", flush=True)
+    out_a, st_a = sweep(c, lams, "auto")
+    out_l, st_l = sweep(c, lams, "lu32ir")
+    assert st_a["pivot_fallbacks"] == 0
+    assert out_a["iterations"][:, 0].max() > out_l["iterations"][:, 0].max()  # GMRES steps, not refinement steps
+    for k in ("x_sca", "x_sca_sh"):
+        for i in range(lams.size):
+            assert rel(out_a[k][i], out_l[k][i]) < 1e-10, (k, i)
+    # against the oracle: x_sca at the mixed bar; x_sca_SH no farther from it
+    # than the mixed LU's (at nmax 8 the SH solution's conditioning spreads
+    # FP64 solvers over about 1e-9; the spread-aware bars are the full-size
+    # tests')
+    ref = ora.solve_all(c.with_wavelength(lams[3]))
+    assert rel(out_a["x_sca"][3], ref["x_sca"]) < 1e-9
+    assert rel(out_a["x_sca_sh"][3], ref["x_sca_sh"]) <= max(1e-9, 2 * rel(out_l["x_sca_sh"][3], ref["x_sca_sh"]))
+
+
+def test_auto_sweep_falls_back_to_the_batched_lu(sweep_case):
+    """One GMRES step per system: every chunk factors S after all and refines
+    from the GMRES iterate; the answers are the mixed LU sweep's."""
+    c, lams = sweep_case
+    out_a, st_a = sweep(c, lams, "auto", budget=1)
+    out_l, st_l = sweep(c, lams, "lu32ir")
+    assert st_a["pivot_fallbacks"] == 2  # both chunks of four
+    for k in ("x_sca", "x_sca_sh"):
+        for i in range(lams.size):
+            assert rel(out_a[k][i], out_l[k][i]) < 1e-11, (k, i)
