@@ -139,6 +139,7 @@ struct omx_context {
   // device buffers
   DBuf d_xyz, d_radius, d_eps, d_mu, d_eps_sh, d_kppp, d_kpar, d_gam, d_sph, d_ov, d_inc, d_incloc, d_Q, d_w,
       d_coef, d_S64, d_S32, d_V64, d_V32, d_x, d_xsh, d_xint, d_xintsh, d_K, d_K1, d_cross, d_cg, d_abscoef, d_abssh,
+      d_absbes,
       d_fpts, d_fout;
   DBuf d_piv, d_perm, d_cand, d_cnt, d_map, d_status, d_flags, d_wlu;
   hipStream_t aux = nullptr;
@@ -166,7 +167,7 @@ struct omx_context {
   // wavelength sweeps: per-system buffers of one batch ([system][...])
   struct Sweep {
     DBuf S64, S32, V64, V32, F64, F64sh, eps, eps_sh, ov, incloc, Q, w, x, xint, xsh, xintsh, K, K1, cross, abscoef,
-        abssh;
+        abssh, absbes;
     DBuf perm, perm2, flags, flags2, dinv, dinv2, map, status, basis, basisZ, r, w1, w2, gpart, part, hd, nrm, sc,
         act, kz, omz, sys, map2, status2;
   } sw;
@@ -331,6 +332,7 @@ void alloc_all(omx_context *c) {
   c->d_cross.ensure(sizeof(double) * 4 * P, T);
   c->d_abscoef.ensure(sizeof(cplx) * P * nf, T);
   c->d_abssh.ensure(sizeof(double) * P, T);
+  c->d_absbes.ensure(sizeof(cplx) * P * omx::kAbsBes, T);
   c->d_r.ensure(sizeof(cplx) * Np, T);
   c->d_w1.ensure(sizeof(cplx) * Np, T);
   c->d_w2.ensure(sizeof(cplx) * Np, T);
@@ -1367,8 +1369,9 @@ void sweep_alloc(omx_context *c, int nb, bool f64, bool sh) {
     b->ensure(sizeof(cplx) * Np * nb, T);
   w.w.ensure(sizeof(double) * 2 * P * c->nf * nb, T);
   w.cross.ensure(sizeof(double) * 4 * P * nb, T);
-  w.abscoef.ensure(sizeof(cplx) * P * c->nf, T);
+  w.abscoef.ensure(sizeof(cplx) * P * c->nf * nb, T);
   w.abssh.ensure(sizeof(double) * P * nb, T);
+  w.absbes.ensure(sizeof(cplx) * P * omx::kAbsBes * nb, T);
   for(DBuf *b : {&w.perm, &w.perm2}) b->ensure(sizeof(int) * Np * nb, T);
   for(DBuf *b : {&w.flags, &w.flags2}) b->ensure(sizeof(int) * (Np / omx::kNB + 1) * nb, T);
   for(DBuf *b : {&w.dinv, &w.dinv2}) b->ensure(sizeof(cplx) * omx::dinv_elems(Np) * nb, T);
@@ -1858,16 +1861,17 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
                       w.cross.as<double>() + (size_t)z * 4 * P};
     omx::run_cross(ca, st);
   }
-  // SH absorption (Result.cpp:763-794), per system; the scratch is reused in stream order
-  if(sh)
-    for(int z = 0; z < nb; ++z) {
-      const double omega = omx::kC0 * (2.0 * omx::kPi / lam[z]);
-      omx::AbsShArgs aa{P, nmax, omega, bge, bgm, c->d_radius.as<double>(), w.eps.as<cplx>() + (size_t)z * P,
-                        c->d_mu.as<cplx>(), w.eps_sh.as<cplx>() + (size_t)z * P, c->d_gam.as<cplx>(),
-                        c->d_cg.as<double>(), w.xint.as<cplx>() + z * lda, w.xintsh.as<cplx>() + z * lda,
-                        w.abscoef.as<cplx>(), w.abssh.as<double>() + (size_t)z * P};
-      omx::run_abssh(aa, st);
-    }
+  // SH absorption (Result.cpp:763-794), every system in one batch
+  if(sh) {
+    omx::AbsShArgs aa{P, nmax, hom[0], bge, bgm, c->d_radius.as<double>(), w.eps.as<cplx>(), c->d_mu.as<cplx>(),
+                      w.eps_sh.as<cplx>(), c->d_gam.as<cplx>(), c->d_cg.as<double>(), w.xint.as<cplx>(),
+                      w.xintsh.as<cplx>(), w.abscoef.as<cplx>(), w.abssh.as<double>()};
+    aa.bes = w.absbes.as<cplx>();
+    aa.omz = w.omz.as<double>();
+    aa.nb = nb;
+    aa.zvec = lda;
+    omx::run_abssh(aa, st);
+  }
   std::vector<double> part((size_t)4 * P * nb), pabs((size_t)P * nb, 0.0);
   HIPCHK(hipMemcpyAsync(part.data(), w.cross.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
   if(sh) HIPCHK(hipMemcpyAsync(pabs.data(), w.abssh.p, sizeof(double) * pabs.size(), hipMemcpyDeviceToHost, st));
@@ -2009,14 +2013,14 @@ void omx_destroy(omx_context *c) {
                   &c->d_K1, &c->d_cross, &c->d_cg, &c->d_piv, &c->d_perm, &c->d_cand, &c->d_cnt, &c->d_map,
                   &c->d_wlu, &c->d_status, &c->d_flags, &c->d_basis, &c->d_basisZ, &c->d_w1, &c->d_w2, &c->d_r,
                   &c->d_part, &c->d_h, &c->d_nrm, &c->d_gpart, &c->d_piv2, &c->d_perm2, &c->d_flags2,
-                  &c->d_dinv, &c->d_dinv2, &c->d_F64, &c->d_F64sh, &c->d_abscoef, &c->d_abssh,
+                  &c->d_dinv, &c->d_dinv2, &c->d_F64, &c->d_F64sh, &c->d_abscoef, &c->d_abssh, &c->d_absbes,
                   &c->d_fpts, &c->d_fout};
   for(DBuf *b : bufs) b->release(&c->total);
   auto &w = c->sw;
   DBuf *sbufs[] = {&w.S64, &w.S32, &w.V64, &w.V32, &w.F64, &w.F64sh, &w.eps, &w.eps_sh, &w.ov, &w.incloc, &w.Q,
                    &w.w, &w.x, &w.xint, &w.xsh, &w.xintsh, &w.K, &w.K1, &w.cross, &w.perm, &w.perm2, &w.flags,
                    &w.flags2, &w.dinv, &w.dinv2, &w.map, &w.status, &w.basis, &w.basisZ, &w.r, &w.w1, &w.w2, &w.gpart,
-                   &w.part, &w.hd, &w.nrm, &w.sc, &w.act, &w.abscoef, &w.abssh, &w.kz, &w.omz};
+                   &w.part, &w.hd, &w.nrm, &w.sc, &w.act, &w.abscoef, &w.abssh, &w.absbes, &w.kz, &w.omz};
   for(DBuf *b : sbufs) b->release(&c->total);
   c->aca[0].release(&c->total);
   c->aca[1].release(&c->total);
@@ -2667,6 +2671,7 @@ int omx_absorption_cross_section_sh(omx_context *c, double *out) {
                       c->d_eps.as<cplx>(), c->d_mu.as<cplx>(), c->d_eps_sh.as<cplx>(), c->d_gam.as<cplx>(),
                       c->d_cg.as<double>(), c->d_xint.as<cplx>(), c->d_xintsh.as<cplx>(), c->d_abscoef.as<cplx>(),
                       c->d_abssh.as<double>()};
+    aa.bes = c->d_absbes.as<cplx>();
     omx::run_abssh(aa, c->st);
     HIPCHK(hipGetLastError());
     std::vector<double> part(c->P);

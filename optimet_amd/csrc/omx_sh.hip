@@ -215,49 +215,78 @@ __global__ __launch_bounds__(256) void shsrc_kernel(ShArgs a) {
 // kept: the second derivative of bessel3der (Bessel.h:230-231, i in place of
 // i + 1) and the SH X(+1) term j_n / r + j_n'(K r) without the factor K
 // (Symbol.cpp:460).
+constexpr int kAbsNQ = 4;
+__device__ const double kAbsXi[kAbsNQ] = {-0.3399810435848563, 0.3399810435848563, -0.8611363115940526,
+                                          0.8611363115940526};
+
+// One thread per (object j, radius i, system z): the Bessel values of the
+// integrand at the radius, for every order, into the object's table
+// ([5][NQ][CAP]: j, j', j'' at k_j r, then j, j' at k_SH r).
+__global__ void abssh_bessel_kernel(AbsShArgs a) {
+  const int z = blockIdx.z, e = blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= a.P * kAbsNQ) return;
+  const int j = e / kAbsNQ, i = e % kAbsNQ;
+  constexpr int CAP = kMaxNmax + 2;
+  const int nmax = a.nmax, S = nmax + 1;
+  const double omega = a.omz ? a.omz[z] : a.omega;
+  const cplx *eps = a.eps + (size_t)z * a.P, *eps_sh = a.eps_sh + (size_t)z * a.P;
+  cplx *tb = a.bes + ((size_t)z * a.P + j) * kAbsBes;
+  const double R = a.radius[j];
+  const cplx kj = omega * csqrt_(eps[j] * a.mu[j]);
+  const cplx ksh = (2.0 * omega) * csqrt_(eps_sh[j] * a.mu[j]);
+  const double r = (R / 2.0) * kAbsXi[i] + R / 2.0;
+  cplx f[CAP], d[CAP];
+  const cplx x = r * kj;
+  sph_bessel(0, x, nmax, f, d);
+  for(int n = 0; n < S; ++n) {
+    tb[(0 * kAbsNQ + i) * CAP + n] = f[n];
+    tb[(1 * kAbsNQ + i) * CAP + n] = d[n];
+    cplx v = mk(0.0);
+    if(n >= 1 && hypot(x.x, x.y) > kErrEps) {
+      v = -(mk((double)n) / x) * d[n] + (mk((double)n) / (x * x)) * f[n] + d[n - 1];
+      if(x.y == 0.0) v.y = 0.0;
+    }
+    tb[(2 * kAbsNQ + i) * CAP + n] = v;
+  }
+  sph_bessel(0, r * ksh, nmax, f, d);
+  for(int n = 0; n < S; ++n) {
+    tb[(3 * kAbsNQ + i) * CAP + n] = f[n];
+    tb[(4 * kAbsNQ + i) * CAP + n] = d[n];
+  }
+}
+
 __global__ __launch_bounds__(256) void abssh_kernel(AbsShArgs a) {
-  const int j = blockIdx.x, kk = blockIdx.y, t = threadIdx.x;
+  const int j = blockIdx.x, kk = blockIdx.y, t = threadIdx.x, z = blockIdx.z;
   const int nmax = a.nmax, nf = nmax * (nmax + 2), S = nmax + 1;
   const size_t sz = (size_t)nf * nf * nf;
-  constexpr int NQ = 4, CAP = kMaxNmax + 2;
+  constexpr int NQ = kAbsNQ, CAP = kMaxNmax + 2;
   __shared__ cplx jb[NQ][CAP], djb[NQ][CAP], d2jb[NQ][CAP], jsh[NQ][CAP], djsh[NQ][CAP];
   __shared__ cplx cs[kMaxNmax * (kMaxNmax + 2)], ds[kMaxNmax * (kMaxNmax + 2)];
   __shared__ cplx red[2 * NQ][256];
-  const double xi[NQ] = {-0.3399810435848563, 0.3399810435848563, -0.8611363115940526, 0.8611363115940526};
   const double wi[NQ] = {0.6521451548625461, 0.6521451548625461, 0.3478548451374538, 0.3478548451374538};
+  const double omega = a.omz ? a.omz[z] : a.omega;
+  const cplx *eps = a.eps + (size_t)z * a.P, *eps_sh = a.eps_sh + (size_t)z * a.P;
+  const cplx *xint = a.xint + z * a.zvec, *xintsh = a.xintsh + z * a.zvec;
   const double R = a.radius[j];
-  const cplx kj = a.omega * csqrt_(a.eps[j] * a.mu[j]);
-  const cplx ksh = (2.0 * a.omega) * csqrt_(a.eps_sh[j] * a.mu[j]);
-  if(t < NQ) {
-    const double r = (R / 2.0) * xi[t] + R / 2.0;
-    cplx f[CAP], d[CAP];
-    const cplx z = r * kj;
-    sph_bessel(0, z, nmax, f, d);
-    for(int n = 0; n < S; ++n) {
-      jb[t][n] = f[n];
-      djb[t][n] = d[n];
-      cplx v = mk(0.0);
-      if(n >= 1 && hypot(z.x, z.y) > kErrEps) {
-        v = -(mk((double)n) / z) * d[n] + (mk((double)n) / (z * z)) * f[n] + d[n - 1];
-        if(z.y == 0.0) v.y = 0.0;
-      }
-      d2jb[t][n] = v;
-    }
-    sph_bessel(0, r * ksh, nmax, f, d);
-    for(int n = 0; n < S; ++n) {
-      jsh[t][n] = f[n];
-      djsh[t][n] = d[n];
+  const cplx kj = omega * csqrt_(eps[j] * a.mu[j]);
+  const cplx ksh = (2.0 * omega) * csqrt_(eps_sh[j] * a.mu[j]);
+  {
+    const cplx *tb = a.bes + ((size_t)z * a.P + j) * kAbsBes;
+    cplx(*dst[5])[CAP] = {jb, djb, d2jb, jsh, djsh};
+    for(int e = t; e < 5 * NQ * S; e += blockDim.x) {
+      const int w = e / (NQ * S), i = (e / S) % NQ, n = e % S;
+      dst[w][i][n] = tb[(w * NQ + i) * CAP + n];
     }
   }
   for(int p = t; p < nf; p += blockDim.x) {
-    cs[p] = a.xint[(size_t)j * 2 * nf + p];
-    ds[p] = a.xint[(size_t)j * 2 * nf + nf + p];
+    cs[p] = xint[(size_t)j * 2 * nf + p];
+    ds[p] = xint[(size_t)j * 2 * nf + nf + p];
   }
   __syncthreads();
   int n, m;
   unflat(kk, n, m);
   const double snn = sqrt((double)n * (n + 1));
-  const cplx pre = (-mk(kEps0) / a.eps_sh[j]) * a.gamma[j];
+  const cplx pre = (-mk(kEps0) / eps_sh[j]) * a.gamma[j];
   const cplx ik2 = mk(1.0) / (kj * kj);
   cplx xm1[NQ], xp1[NQ];
   for(int i = 0; i < NQ; ++i) xm1[i] = xp1[i] = mk(0.0);
@@ -277,7 +306,7 @@ __global__ __launch_bounds__(256) void abssh_kernel(AbsShArgs a) {
     const cplx cc = cs[p] * cs[q], dd = ds[p] * ds[q];
 #pragma unroll
     for(int i = 0; i < NQ; ++i) {
-      const double r = (R / 2.0) * xi[i] + R / 2.0, ir = 1.0 / r;
+      const double r = (R / 2.0) * kAbsXi[i] + R / 2.0, ir = 1.0 / r;
       const cplx j1 = jb[i][n1], j2 = jb[i][n2], dj1 = djb[i][n1], dj2 = djb[i][n2];
       // F_d00, F_d11, F_dm1m1, F_00, F_11, F_m1m1 (Symbol.cpp:80-141)
       const cplx fd00 = j1 * dj2 * kj + kj * dj1 * j2;
@@ -303,32 +332,33 @@ __global__ __launch_bounds__(256) void abssh_kernel(AbsShArgs a) {
     __syncthreads();
   }
   if(t == 0) {
-    const cplx k01 = mk(a.omega * sqrt(kEps0 * kMu0));
-    const cplx csh = a.xintsh[(size_t)j * 2 * nf + kk], dsh = a.xintsh[(size_t)j * 2 * nf + nf + kk];
+    const cplx k01 = mk(omega * sqrt(kEps0 * kMu0));
+    const cplx csh = xintsh[(size_t)j * 2 * nf + kk], dsh = xintsh[(size_t)j * 2 * nf + nf + kk];
     cplx integral = mk(0.0);
     for(int i = 0; i < NQ; ++i) {
-      const double r = (R / 2.0) * xi[i] + R / 2.0;
+      const double r = (R / 2.0) * kAbsXi[i] + R / 2.0;
       const cplx xm1sh = k01 * dsh * (mk(1.0) / ksh) * snn * (1.0 / r) * jsh[i][n];
       const cplx x0 = -(k01 * csh * jsh[i][n]);
       const cplx xp1sh = k01 * dsh * (mk(1.0) / ksh) * ((1.0 / r) * jsh[i][n] + djsh[i][n]);
       const cplx em = red[i][0] + xm1sh, ep = red[NQ + i][0] + xp1sh;
       integral = integral + (wi[i] * (r * r)) * (em * conj(em) + x0 * conj(x0) + ep * conj(ep));
     }
-    a.coef[(size_t)j * nf + kk] = integral * (R / 2.0);
+    a.coef[((size_t)z * a.P + j) * nf + kk] = integral * (R / 2.0);
   }
 }
 
 // Per object: Re((2 eta) 0.5 sigma_j sum_kk coefABS), harmonics summed in order.
 __global__ void abssh_sum_kernel(AbsShArgs a) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, z = blockIdx.z;
   if(j >= a.P) return;
   const int nf = a.nmax * (a.nmax + 2);
+  const double omega = a.omz ? a.omz[z] : a.omega;
   cplx s = mk(0.0);
-  for(int kk = 0; kk < nf; ++kk) s = s + a.coef[(size_t)j * nf + kk];
+  for(int kk = 0; kk < nf; ++kk) s = s + a.coef[((size_t)z * a.P + j) * nf + kk];
   const cplx eta = csqrt_(a.bg_mu / a.bg_eps);
-  const cplx eps_r_sh = a.eps_sh[j] / mk(kEps0);
-  const cplx sigma = -(mk(0.0, 1.0) * mk(kEps0) * (2.0 * a.omega)) * (eps_r_sh - mk(1.0));
-  a.out[j] = ((mk(2.0) * eta) * mk(0.5) * sigma * s).x;
+  const cplx eps_r_sh = a.eps_sh[(size_t)z * a.P + j] / mk(kEps0);
+  const cplx sigma = -(mk(0.0, 1.0) * mk(kEps0) * (2.0 * omega)) * (eps_r_sh - mk(1.0));
+  a.out[(size_t)z * a.P + j] = ((mk(2.0) * eta) * mk(0.5) * sigma * s).x;
 }
 
 void upload_factorials() {
@@ -346,8 +376,9 @@ void run_cg(int nmax, double *out, hipStream_t st) {
 
 void run_abssh(const AbsShArgs &a, hipStream_t st) {
   int nf = a.nmax * (a.nmax + 2);
-  hipLaunchKernelGGL(abssh_kernel, dim3(a.P, nf), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(abssh_sum_kernel, dim3((a.P + 63) / 64), dim3(64), 0, st, a);
+  hipLaunchKernelGGL(abssh_bessel_kernel, dim3((a.P * kAbsNQ + 63) / 64, 1, a.nb), dim3(64), 0, st, a);
+  hipLaunchKernelGGL(abssh_kernel, dim3(a.P, nf, a.nb), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(abssh_sum_kernel, dim3((a.P + 63) / 64, 1, a.nb), dim3(64), 0, st, a);
 }
 
 void run_shsrc(const ShArgs &a, hipStream_t st) {

@@ -31,7 +31,16 @@ struct AbsShArgs {
   const cplx *xint, *xintsh;     // internal coefficients at omega, 2 omega (N each)
   cplx *coef;                    // scratch, P nf
   double *out;                   // P
+  // the Bessel values of the four quadrature radii per object (j, dj, d2j at
+  // k_j r; j, dj at k_SH r), kAbsBes complex each, computed once per object
+  cplx *bes = nullptr;
+  // batches (blockIdx.z = system z): omega omz[z], eps, eps_sh + z P,
+  // xint, xintsh + z zvec, coef + z P nf, out + z P, bes + z P kAbsBes
+  const double *omz = nullptr;
+  int nb = 1;
+  size_t zvec = 0;
 };
+constexpr int kAbsBes = 4 * 5 * (kMaxNmax + 2);
 
 void upload_factorials();
 void run_cg(int nmax, double *out, hipStream_t st);
