@@ -324,7 +324,12 @@ __device__ inline void pair_vector(const double *sph, int i, int j, double &r, d
 }
 
 // ------------------------------------------------------------------ assembly
-// One block per ordered pair (i = block row, j = block column).
+// One block per unordered pair i <= j (block rows i and j, triangular index
+// blockIdx.x), or per listed ordered pair (pair-list mode, one block each, in
+// its own slot).  The coefficients are always those of the pair (min, max);
+// S_ji with j > i comes from them by the parity of the translation: A_pq(-R) = (-1)^(n+l) A_pq(R), B_pq(-R) = -(-1)^(n+l) B_pq(R)
+// for p of degree n and q of degree l, scaled by -T_j (the reference computes
+// the ordered pair j, i through its own pair vector, equal to rounding).
 template <int NMAX>
 __global__ __launch_bounds__(256) void pair_kernel(PairArgs a) {
   constexpr int NF = PairLDS<NMAX>::NF;
@@ -337,19 +342,35 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs a) {
     if(a.S64) a.S64 += z * a.zS;
     if(a.S32) a.S32 += z * a.zS;
   }
-  int pid = blockIdx.x;
+  int i, j;
   size_t x0, y0;
   if(a.plist) {  // block blockIdx.x of a pair list, in its own slot
-    pid = a.plist[blockIdx.x];
+    const int pid = a.plist[blockIdx.x];
+    i = pid / P;
+    j = pid % P;
     if(a.S64) a.S64 += blockIdx.x * a.zblk;
     if(a.S32) a.S32 += blockIdx.x * a.zblk;
+  } else {  // triangular index k = j (j + 1) / 2 + i, i <= j
+    const int k = blockIdx.x;
+    int jj = (int)((sqrt(8.0 * k + 1.0) - 1.0) * 0.5);
+    while((jj + 1) * (jj + 2) / 2 <= k) ++jj;
+    while(jj * (jj + 1) / 2 > k) --jj;
+    j = jj;
+    i = k - jj * (jj + 1) / 2;
   }
-  const int i = pid / P, j = pid % P;
+  // the coefficients are those of the pair (ci, cj), ci <= cj; a listed block
+  // with i > j is written as the mirror, as the full assembly writes it
+  const int ci = min(i, j), cj = max(i, j);
+  const bool direct = !a.plist || i < j;
+  const bool mirror = a.plist ? i > j : i != j;
+  size_t x1, y1;  // the mirrored block S_{cj ci}
   if(a.plist) {
-    x0 = y0 = 0;
+    x0 = y0 = x1 = y1 = 0;
   } else {
-    x0 = (size_t)i * 2 * NF;
-    y0 = (size_t)j * 2 * NF;
+    x0 = (size_t)ci * 2 * NF;
+    y0 = (size_t)cj * 2 * NF;
+    x1 = y0;
+    y1 = x0;
   }
   const int t = threadIdx.x;
   const int cg = blockDim.x / NF;  // column groups
@@ -367,19 +388,26 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs a) {
     }
     return;
   }
-  // row scale -T_i
+  // row scale -T_ci
   for(int r = t; r < 2 * NF; r += blockDim.x) {
     int nn, mm;
     unflat(r % NF, nn, mm);
-    s.Ti[r] = -a.ov[(((size_t)a.tkind * P + i) * 2 + (r >= NF)) * (NMAX + 1) + nn];
+    s.Ti[r] = -a.ov[(((size_t)a.tkind * P + ci) * 2 + (r >= NF)) * (NMAX + 1) + nn];
   }
   double Rr, Rt, Rp;
-  pair_vector(a.sph, i, j, Rr, Rt, Rp);
+  pair_vector(a.sph, ci, cj, Rr, Rt, Rp);
   const bool ident = fabs(Rr) < kErrEps;  // Coupling.cpp:82-84
   if(!ident) pair_setup<NMAX>(s, Rr, Rt, Rp, a.k, true);
   __syncthreads();
   QFac f = qfac(q);
   const cplx Tq = s.Ti[q], Tq2 = s.Ti[q + NF];
+  // row scale -T_cj of the mirrored block, and this row's parity (-1)^l
+  cplx Uq = mk(0.0), Uq2 = mk(0.0);
+  if(mirror && qact) {
+    Uq = -a.ov[(((size_t)a.tkind * P + cj) * 2 + 0) * (NMAX + 1) + f.l];
+    Uq2 = -a.ov[(((size_t)a.tkind * P + cj) * 2 + 1) * (NMAX + 1) + f.l];
+  }
+  const double sq = (f.l & 1) ? -1.0 : 1.0;
   for(int n = 1; n <= NMAX; ++n) {
     if(!ident) build_table<NMAX>(s, n, true, a.coef);
     if(qact) {
@@ -394,17 +422,35 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs a) {
         }
         cplx v00 = Tq * A, v10 = Tq2 * B, v01 = Tq * B, v11 = Tq2 * A;
         size_t c0 = (y0 + p) * a.lda + x0, c1 = (y0 + NF + p) * a.lda + x0;
-        if(a.S64) {
+        if(direct && a.S64) {
           a.S64[c0 + q] = v00;
           a.S64[c0 + NF + q] = v10;
           a.S64[c1 + q] = v01;
           a.S64[c1 + NF + q] = v11;
         }
-        if(a.S32) {
+        if(direct && a.S32) {
           a.S32[c0 + q] = make_float2((float)v00.x, (float)v00.y);
           a.S32[c0 + NF + q] = make_float2((float)v10.x, (float)v10.y);
           a.S32[c1 + q] = make_float2((float)v01.x, (float)v01.y);
           a.S32[c1 + NF + q] = make_float2((float)v11.x, (float)v11.y);
+        }
+        if(mirror) {
+          const double sa = (n & 1) ? -sq : sq;
+          const cplx Am = sa * A, Bm = (-sa) * B;
+          const cplx w00 = Uq * Am, w10 = Uq2 * Bm, w01 = Uq * Bm, w11 = Uq2 * Am;
+          const size_t d0 = (y1 + p) * a.lda + x1, d1 = (y1 + NF + p) * a.lda + x1;
+          if(a.S64) {
+            a.S64[d0 + q] = w00;
+            a.S64[d0 + NF + q] = w10;
+            a.S64[d1 + q] = w01;
+            a.S64[d1 + NF + q] = w11;
+          }
+          if(a.S32) {
+            a.S32[d0 + q] = make_float2((float)w00.x, (float)w00.y);
+            a.S32[d0 + NF + q] = make_float2((float)w10.x, (float)w10.y);
+            a.S32[d1 + q] = make_float2((float)w01.x, (float)w01.y);
+            a.S32[d1 + NF + q] = make_float2((float)w11.x, (float)w11.y);
+          }
         }
       }
     }
@@ -539,7 +585,7 @@ __global__ void pad_kernel(T *A, int N, int Np, size_t lda, size_t zs) {
 // ------------------------------------------------------------------ launchers
 template <int NMAX>
 static void launch_pair(const PairArgs &a, hipStream_t st, int nb) {
-  hipLaunchKernelGGL(pair_kernel<NMAX>, dim3(a.plist ? a.nlist : a.P * a.P, 1, nb), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(pair_kernel<NMAX>, dim3(a.plist ? a.nlist : a.P * (a.P + 1) / 2, 1, nb), dim3(256), 0, st, a);
 }
 template <int NMAX>
 static void launch_origin(const OriginArgs &a, hipStream_t st, int nb) {
