@@ -79,19 +79,39 @@ template <typename T> __device__ inline T csub(T a, T b) {
   return r;
 }
 template <typename T> __device__ inline double mag2(T a) { return (double)a.x * a.x + (double)a.y * a.y; }
+// Reciprocal: the hardware estimate refined by Newton's iteration
+// r <- r + r (1 - d r), which doubles the number of correct bits per step:
+// one step after the single-precision estimate (within 1 ulp), two after the
+// double-precision one (about 23 correct bits).
+__device__ inline float rcp_nr(float d) {
+  const float r = __builtin_amdgcn_rcpf(d);
+  return __builtin_fmaf(r, __builtin_fmaf(-d, r, 1.0f), r);
+}
+__device__ inline double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+  return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+}
+__device__ inline float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fmx(double a, double b, double c) { return __builtin_fma(a, b, c); }
 template <typename T> __device__ inline T crecip(T a) {
-  // 1 / a, scaled to avoid overflow
+  // 1 / a by Smith's scaling: with s the smaller and b the larger component
+  // in magnitude, q = s / b, 1 / a = (1, -q) / (b + s q) for |re| >= |im| and
+  // (q, -1) / (b + s q) otherwise, so no intermediate overflows
   using R = decltype(a.x);
+  const bool re = fabs(a.x) >= fabs(a.y);
+  const R b = re ? a.x : a.y, sm = re ? a.y : a.x;
+  const R q = sm * rcp_nr(b), id = rcp_nr(fmx(sm, q, b));
   T r;
-  if(fabs((double)a.x) >= fabs((double)a.y)) {
-    R q = a.y / a.x, d = a.x + a.y * q;
-    r.x = R(1) / d;
-    r.y = -q / d;
-  } else {
-    R q = a.x / a.y, d = a.x * q + a.y;
-    r.x = q / d;
-    r.y = R(-1) / d;
-  }
+  r.x = re ? id : q * id;
+  r.y = re ? -q * id : -id;
+  return r;
+}
+// v - a b with fused multiply-adds
+template <typename R> __device__ inline Cx<R> cfms(Cx<R> v, Cx<R> a, Cx<R> b) {
+  Cx<R> r;
+  r.x = fmx(-a.x, b.x, fmx(a.y, b.y, v.x));
+  r.y = fmx(-a.x, b.y, fmx(-a.y, b.x, v.y));
   return r;
 }
 
@@ -418,13 +438,13 @@ __device__ inline unsigned wave_max_u32(unsigned x) {
 // Diagonal-block pivoting: LU with partial pivoting restricted to the 64 x 64
 // diagonal block of the panel (rows are chosen among the block's rows; the
 // interchanges are applied to whole rows), then the inverses of L11 and U11.
-// One workgroup of 8 waves.  Elimination: wave q holds columns 8q..8q+7 of
-// every row in registers, lane = row; rows are not moved (the pivot order is
-// recorded instead).  Per column the owning wave finds the pivot with a DPP
-// maximum of a 32-bit key (max(|re|, |im|) as a float with the row in the
-// low bits) and publishes the column; after one barrier every wave forms the
-// multipliers itself (with the pivot's reciprocal from the owner) and
-// updates its columns with the pivot row read from its own lane b.  Inversion: waves 0-3 build L11^{-1}, waves 4-7 U11^{-1}
+// One workgroup of 8 waves.  Elimination: each wave holds one panel of 8
+// columns of every row in registers, lane = row; rows are not moved (the
+// pivot order is recorded instead).  The panel's owner runs its 8 pivot steps
+// alone (pivot by a DPP maximum of a 32-bit key: max(|re|, |im|) as a float
+// with the row in the low bits) and publishes the multipliers and pivot rows;
+// after one barrier per panel the waves of later panels apply its 8 rank-1
+// updates, each with the pivot row read from its own lane b.  Inversion: waves 0-3 build L11^{-1}, waves 4-7 U11^{-1}
 // (lane = row, 16 columns per wave, row k broadcast by lane reads).  Writes
 // the factored block into A, the inverses into dinv ([L11^{-1} | U11^{-1}],
 // column-major), the block's row map (mdst <- msrc) and the permutation.
@@ -442,64 +462,87 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     perm += (size_t)z * bs.perm;
     status += (size_t)z * bs.status;
   }
-  __shared__ C colc[2][64];
-  __shared__ C ipc[2];  // reciprocal of the pivot
-  __shared__ int pbs[2];
+  __shared__ C lm[2][8][64];  // a panel's multipliers, one column per pivot step (0 on rows already chosen)
+  __shared__ int pb[2][8];    // a panel's pivot rows
   __shared__ int prow[64];
   __shared__ C F[64 * 65];  // factored block in pivoted order, column-major [c * 65 + r]
   const int t = threadIdx.x, lane = t & 63, q = t >> 6;
   const C zero{}, one{1, 0};
   {
+    // Wave q holds the columns of panel pq (8 columns); panels are taken in
+    // order 0..7, and the two waves of a SIMD (q and q + 4) hold consecutive
+    // panels, so every second panel's owner has its SIMD to itself.
+    const int pq = 2 * (q & 3) + (q >> 2);
     C v[8];
 #pragma unroll
     for(int j = 0; j < 8; ++j) {
-      T a = A[(size_t)(j0 + 8 * q + j) * lda + j0 + lane];
+      T a = A[(size_t)(j0 + 8 * pq + j) * lda + j0 + lane];
       v[j] = C{a.x, a.y};
     }
     bool used = false;
     int mypos = 0;
 #pragma unroll 1
-    for(int qc = 0; qc < 8; ++qc) {
+    for(int pc = 0; pc < 8; ++pc) {
+      const int par = pc & 1;
+      if(pq == pc) {
+        // the owner's 8 pivot steps, without barriers: the pivot is the
+        // largest max(|re|, |im|) among the rows not chosen yet (a DPP maximum
+        // of a 32-bit key with the row in the low bits); the multipliers
+        // replace the column below the pivot and update the panel's later
+        // columns, and are published for the other waves
 #pragma unroll
-      for(int jj = 0; jj < 8; ++jj) {
-        const int c = 8 * qc + jj, par = jj & 1;
-        if(q == qc) {
+        for(int jj = 0; jj < 8; ++jj) {
+          const int c = 8 * pc + jj;
           const float mx = (float)fmax(fabs((double)v[jj].x), fabs((double)v[jj].y));
           const unsigned key = used ? 0u : ((__float_as_uint(mx) & ~0x7Fu) | (unsigned)(127 - lane));
-          const unsigned best = wave_max_u32(key);
-          const int bq = 127 - (int)(best & 127);
-          colc[par][lane] = v[jj];
-          if(lane == bq) {
-            const bool sing = !(mag2(v[jj]) > 0.0);
-            if(sing) atomicCAS(&status[0], 0, j0 + c + 1);
-            pbs[par] = bq;
-            ipc[par] = csel(sing, zero, crecip(v[jj]));
+          const int b = __builtin_amdgcn_readfirstlane(127 - (int)(wave_max_u32(key) & 127));
+          const C pv = rdl(v[jj], b);
+          const bool sing = !(mag2(pv) > 0.0);
+          if(sing && lane == b) atomicCAS(&status[0], 0, j0 + c + 1);
+          const C ip = csel(sing, zero, crecip(pv));
+          if(lane == b) {
+            used = true;
+            mypos = c;
+          }
+          const C l = csel(used, zero, cm(v[jj], ip));
+          v[jj] = csel(used, v[jj], l);
+          lm[par][jj][lane] = l;
+          if(lane == 0) {
+            pb[par][jj] = b;
+            prow[c] = b;
+          }
+#pragma unroll
+          for(int j = jj + 1; j < 8; ++j) v[j] = cfms(v[j], l, rdl(v[j], b));
+        }
+      }
+      // one barrier per panel; the buffers alternate, so the next owner's
+      // writes cannot reach a panel that a slower wave still reads
+      __syncthreads();
+      if(pq != pc) {
+        int bb[8];
+#pragma unroll
+        for(int jj = 0; jj < 8; ++jj) {
+          bb[jj] = __builtin_amdgcn_readfirstlane(pb[par][jj]);
+          if(lane == bb[jj]) {
+            used = true;
+            mypos = 8 * pc + jj;
           }
         }
-        __syncthreads();
-        const int b = __builtin_amdgcn_readfirstlane(pbs[par]);
-        const C cv = colc[par][lane];
-        const C ip = ipc[par];
-        if(lane == b) {
-          used = true;
-          mypos = c;
-        }
-        if(t == 0) prow[c] = b;
-        const C l = csel(used, zero, cm(cv, ip));
-        if(q == qc) {
-          v[jj] = csel(used, v[jj], l);
+        if(pq > pc) {
+          // the panel's 8 rank-1 updates, in the order of its pivot steps
 #pragma unroll
-          for(int j = jj + 1; j < 8; ++j) v[j] = csub(v[j], cm(l, rdl(v[j], b)));
-        } else if(q > qc) {
+          for(int jj = 0; jj < 8; ++jj) {
+            const C l = lm[par][jj][lane];
 #pragma unroll
-          for(int j = 0; j < 8; ++j) v[j] = csub(v[j], cm(l, rdl(v[j], b)));
+            for(int j = 0; j < 8; ++j) v[j] = cfms(v[j], l, rdl(v[j], bb[jj]));
+          }
         }
       }
     }
     // factored block in pivoted order
 #pragma unroll
     for(int j = 0; j < 8; ++j) {
-      const int col = 8 * q + j;
+      const int col = 8 * pq + j;
       F[col * 65 + mypos] = v[j];
       T o;
       o.x = v[j].x;
@@ -535,12 +578,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     // L11^{-1} (unit lower): for k: row i -= L[i][k] row k, i > k.  Column
     // 16 w + m starts as e_{16 w + m} and its rows above 16 w + m stay zero,
     // so the steps k < 16 w leave this wave's columns as they are: skipped
+    C fn = F[(16 * w) * 65 + i];  // column k of L, read one step ahead
 #pragma unroll 1
     for(int k = 16 * w; k < 63; ++k) {
       const int ks = __builtin_amdgcn_readfirstlane(k);
-      const C lik = csel(i > ks, F[ks * 65 + i], zero);
+      const C lik = csel(i > ks, fn, zero);
+      fn = F[(ks + 1) * 65 + i];
 #pragma unroll
-      for(int m = 0; m < 16; ++m) x[m] = csub(x[m], cm(lik, rdl(x[m], ks)));
+      for(int m = 0; m < 16; ++m) x[m] = cfms(x[m], lik, rdl(x[m], ks));
     }
   } else {
     // U11^{-1} = (D^{-1} U11)^{-1} D^{-1} with D = diag(U11): a unit-upper
@@ -549,12 +594,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     // columns stay zero, so the sweep starts at k = 16 w + 15
     const C di = F[i * 65 + i];
     const C rinv = csel(mag2(di) > 0.0, crecip(di), zero);
+    C fn = F[(16 * w + 15) * 65 + i];  // column k of U, read one step ahead
 #pragma unroll 1
     for(int k = 16 * w + 15; k > 0; --k) {
       const int ks = __builtin_amdgcn_readfirstlane(k);
-      const C uik = csel(i < ks, cm(F[ks * 65 + i], rinv), zero);
+      const C uik = csel(i < ks, cm(fn, rinv), zero);
+      fn = F[(ks - 1) * 65 + i];
 #pragma unroll
-      for(int m = 0; m < 16; ++m) x[m] = csub(x[m], cm(uik, rdl(x[m], ks)));
+      for(int m = 0; m < 16; ++m) x[m] = cfms(x[m], uik, rdl(x[m], ks));
     }
 #pragma unroll
     for(int m = 0; m < 16; ++m) x[m] = cm(x[m], rdl(rinv, 16 * w + m));
