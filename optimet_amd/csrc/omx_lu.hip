@@ -462,10 +462,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     perm += (size_t)z * bs.perm;
     status += (size_t)z * bs.status;
   }
-  __shared__ C lm[2][8][64];  // a panel's multipliers, one column per pivot step (0 on rows already chosen)
-  __shared__ int pb[2][8];    // a panel's pivot rows
+  __shared__ int pb[2][8];  // a panel's pivot rows
   __shared__ int prow[64];
   __shared__ C F[64 * 65];  // factored block in pivoted order, column-major [c * 65 + r]
+  // A panel's multipliers, one column per pivot step (0 on rows already
+  // chosen), in two alternating buffers.  They live in F's storage: F is
+  // written only after the last panel's barrier, when no wave reads them any
+  // more, and the kernel's LDS stays small enough to start beside the bulk
+  // update's workgroups.
+  C(*lm)[8][64] = reinterpret_cast<C(*)[8][64]>(F);
   const int t = threadIdx.x, lane = t & 63, q = t >> 6;
   const C zero{}, one{1, 0};
   {
