@@ -68,7 +68,10 @@ extern "C" {
                                   OMX_SOLVER_LU32_IR on the same matrices; smaller systems use
                                   OMX_SOLVER_LU32_IR, except that with the second harmonic S
                                   is first tried with FP64 GMRES (within N / 125 products)
-                                  while V's mixed LU factorises (sweeps always batch the LU) */
+                                  while V's mixed LU factorises (from half of V's columns
+                                  on); batched sweeps from N = 2048 with the second harmonic
+                                  try S by lockstep FP64 GMRES (within 2 N / 125 steps)
+                                  beside V's batched mixed LU */
 
 /* Pivot search of the LU factorisations (omx_set_pivoting). */
 #define OMX_PIVOT_BLOCK 0     /* among the rows of each 64 x 64 diagonal block, then

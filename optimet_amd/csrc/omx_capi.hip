@@ -127,6 +127,9 @@ struct omx_context {
   // fraction of S's columns factored before V's factorisation starts
   // (OMX_SH_OVERLAP overrides; 0 starts both together)
   double sh_overlap = 0.5;
+  // OMX_SOLVER_AUTO with S by GMRES: fraction of V's columns factored before
+  // S's GMRES starts (OMX_KRYLOV_MARK overrides; 0 starts both together)
+  double krylov_mark = 0.5;
   int lu32_gemm = OMX_GEMM_BF16X3;
   int fallbacks = 0;
   double tol = 1e-16;
@@ -1973,6 +1976,7 @@ int omx_create(omx_context **out, int device) {
     if(device < 0 || device >= n) throw OmxError(OMX_ERR_DEVICE, "no HIP device " + std::to_string(device));
     auto *c = new omx_context();
     if(const char *e = std::getenv("OMX_SH_OVERLAP")) c->sh_overlap = std::atof(e);
+    if(const char *e = std::getenv("OMX_KRYLOV_MARK")) c->krylov_mark = std::atof(e);
     c->device = device;
     try {
       HIPCHK(hipSetDevice(device));
@@ -2363,8 +2367,28 @@ int omx_solve(omx_context *c, double *x_sca, double *x_int, double *x_sca_sh, do
         c->lu.ev_mark = nullptr;
         c->lu.on_mark = nullptr;
       } else if(c->krylov_s && !c->factored) {
+        // S's GMRES starts once V's factorisation has passed krylov_mark of
+        // its columns: the first columns carry most of the bulk-update work,
+        // which the GMRES products would only slow down, and the panel-bound
+        // rest leaves the device room for them
         HIPCHK(hipEventRecord(c->xev[0], c->st));
-        vq.start(c, start_v);  // S's GMRES starts at once beside it
+        const int kmark = (int)(c->krylov_mark * c->Np);
+        if(kmark > 0) {
+          c->lu_sh.ev_mark = c->xev[3];
+          c->lu_sh.mark_col = kmark;
+          c->lu_sh.on_mark = nullptr;
+        }
+        vq.start(c, start_v);
+        if(kmark > 0) {
+          try {
+            vq.join();  // V's factorisation enqueued, with its mark recorded
+          } catch(...) {
+            c->lu_sh.ev_mark = nullptr;
+            throw;
+          }
+          c->lu_sh.ev_mark = nullptr;
+          HIPCHK(hipStreamWaitEvent(c->st, c->xev[3], 0));
+        }
       } else {
         HIPCHK(hipEventRecord(c->xev[0], c->st));
         start_v(c);
