@@ -33,7 +33,7 @@ BF16_MFMA_TFLOPS = 2500.0  # MI355X dense bf16 matrix peak
 SPLIT_TFLOPS = BF16_MFMA_TFLOPS / 6
 FP64_MFMA_TFLOPS = 78.6    # MI355X dense FP64 matrix peak
 HBM_GBS = 8000.0
-GEMM_KERNEL = {"lu32ir": "gemm_split<128,128,128,4,2,16,8,true> (bf16x3 split, complex FP32, double-buffered split planes; next-pair 128 columns on gemm_tile<float2,32,32>)",
+GEMM_KERNEL = {"lu32ir": "gemm_split<128,128,128,4,2,16,8,true,true> (bf16x3 split, complex FP32, double-buffered split planes, nontemporal C; next-pair 128 columns on gemm_tile<float2,32,32>)",
                "lu64": "gemm_tile<double2,128,64,128,4,2,16> (next-pair 128 columns on gemm_tile<double2,32,32>)"}
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "traffic_gemm_split_config2.txt")

@@ -132,6 +132,30 @@ __device__ inline double rdl1(double v, int l) {
 __device__ inline float rdl1(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
 template <typename R> __device__ inline Cx<R> rdl(Cx<R> v, int l) { return Cx<R>{rdl1(v.x, l), rdl1(v.y, l)}; }
 
+// Loads and stores with the nontemporal hint, for data a kernel streams once
+// (a product's matrix, a bulk update's C tile): they stay out of the caches'
+// replacement, which keeps the reused operands there.  In the FP64 product
+// this raised 6.0 to 6.8 TB/s (N = 10240) and 5.6 to 6.0 TB/s (64 systems of
+// N = 3840).
+__device__ inline cplx ld_stream(const cplx *p) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2 v = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(p));
+  return cplx{v.x, v.y};
+}
+__device__ inline float2 ld_stream(const float2 *p) {
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  const f2 v = __builtin_nontemporal_load(reinterpret_cast<const f2 *>(p));
+  float2 r;
+  r.x = v.x;
+  r.y = v.y;
+  return r;
+}
+__device__ inline void st_stream(float2 *p, float2 c) {
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  const f2 v = {c.x, c.y};
+  __builtin_nontemporal_store(v, reinterpret_cast<f2 *>(p));
+}
+
 __global__ void iota_kernel(int *p, int n, int sp) {
   p += (size_t)blockIdx.z * sp;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1038,7 +1062,7 @@ __device__ inline void split_bf16x3(const float *f, u32x4 &h, u32x4 &m, u32x4 &l
 // DB: the split planes double-buffered in LDS, so that each wave splits and
 // stores chunk c + 1 while its products of chunk c run (one barrier per
 // chunk); the products and their order are those of the single buffer.
-template <int TM, int TN, int KD, int WGM, int WGN, int BK, int GM, bool DB = false>
+template <int TM, int TN, int KD, int WGM, int WGN, int BK, int GM, bool DB = false, bool NTC = false>
 __global__ __launch_bounds__(64 * WGM * WGN) void gemm_split(float2 *A, size_t lda, int row0, int col0, int k0, int M,
                                                             int N, size_t sA) {
   A += blockIdx.z * sA;
@@ -1117,7 +1141,8 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_split(float2 *A, size_t l
           vi[a][b][v] = 0;
           if(ok) {
             const int n = nb + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4), m = mb + (lane & 31);
-            const float2 c = A[(size_t)(col0 + n) * lda + row0 + m];
+            const float2 *cp = A + (size_t)(col0 + n) * lda + row0 + m;
+            const float2 c = NTC ? ld_stream(cp) : *cp;
             vr[a][b][v] = c.x;
             vi[a][b][v] = c.y;
           }
@@ -1212,7 +1237,10 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_split(float2 *A, size_t l
         float2 c;
         c.x = vr[a][b][v] - accR[a][b][v];
         c.y = vi[a][b][v] - accI[a][b][v];
-        A[(size_t)(col0 + n) * lda + row0 + m] = c;
+        if constexpr(NTC)
+          st_stream(A + (size_t)(col0 + n) * lda + row0 + m, c);
+        else
+          A[(size_t)(col0 + n) * lda + row0 + m] = c;
       }
     }
 }
@@ -1225,6 +1253,9 @@ struct SplitTile {
   // double-buffered split planes: bit-identical, 149 against 125 TFLOP/s at
   // M = N = 10112 (one matrix), 156 against 154 for 64 systems of 3712
   static constexpr bool DB = true;
+  // C's tile read and written with the nontemporal hint: bit-identical, 5510
+  // against 5644 us for 64 systems of 3840 (2.4%), 1-5% at one matrix of 10240
+  static constexpr bool NTC = true;
 };
 
 // The bulk update with K = 2 NB: split-bf16 for FP32 factors when `split`,
@@ -1243,7 +1274,7 @@ void bulk_update(T *A, size_t lda, int row0, int col0, int k0, int M, int N, hip
     if(split) {
       using U = SplitTile;
       const int tiles = ((M + U::TM - 1) / U::TM) * ((N + U::TN - 1) / U::TN);
-      hipLaunchKernelGGL((gemm_split<U::TM, U::TN, 2 * kNB, U::WGM, U::WGN, U::BK, U::GM, U::DB>), dim3(tiles, 1, nb),
+      hipLaunchKernelGGL((gemm_split<U::TM, U::TN, 2 * kNB, U::WGM, U::WGN, U::BK, U::GM, U::DB, U::NTC>), dim3(tiles, 1, nb),
                          dim3(64 * U::WGM * U::WGN), 0, st, A, lda, row0, col0, k0, M, N, sA);
       return;
     }
@@ -1740,23 +1771,6 @@ template void lu_solve<float2>(const float2 *, int, size_t, LuWork &, const cplx
 // y = A x for the FP64 matrix (residuals and GMRES) or its complex-FP32 copy
 // (entries widened to FP64, FP64 vectors and sums), two-stage column split.
 constexpr int kGemvSplit = 32;
-// A matrix entry loaded with the nontemporal hint: each product streams the
-// matrix once, and keeping it out of the caches' replacement raised the FP64
-// product from 6.0 to 6.8 TB/s (N = 10240) and from 5.6 to 6.0 TB/s (64
-// systems of N = 3840)
-__device__ inline cplx ld_stream(const cplx *p) {
-  typedef double d2 __attribute__((ext_vector_type(2)));
-  const d2 v = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(p));
-  return cplx{v.x, v.y};
-}
-__device__ inline float2 ld_stream(const float2 *p) {
-  typedef float f2 __attribute__((ext_vector_type(2)));
-  const f2 v = __builtin_nontemporal_load(reinterpret_cast<const f2 *>(p));
-  float2 r;
-  r.x = v.x;
-  r.y = v.y;
-  return r;
-}
 template <typename T>
 __global__ __launch_bounds__(256) void gemv_part_kernel(const T *__restrict__ A, size_t lda, int N,
                                                         const cplx *__restrict__ x, cplx *part, int chunk, size_t sA,
