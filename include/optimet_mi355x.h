@@ -316,7 +316,7 @@ int omx_dense_solve(omx_context *ctx, int N, const double *A, const double *b, d
  * column-major complex128 array with Np = N rounded up to 64 and lda >= Np,
  * whose leading N x N block holds the matrix; the call overwrites A with the
  * identity-padded matrix's LU factors.  b and x hold N complex128 values.
- * Every work array (pivots, permutations, flags, diagonal-block inverses,
+ * Every work array (pivots, permutations, status words, diagonal-block inverses,
  * staged right-hand side) lives in one library-held scratch array per device
  * (about 8.6 MiB, held until the process ends), so N is limited to
  * 1 <= N <= 4096 (OMX_ERR_ARG beyond).  Calls that overlap on one device share

@@ -144,7 +144,7 @@ struct omx_context {
       d_coef, d_S64, d_S32, d_V64, d_V32, d_x, d_xsh, d_xint, d_xintsh, d_K, d_K1, d_cross, d_cg, d_abscoef, d_abssh,
       d_absbes,
       d_fpts, d_fout;
-  DBuf d_piv, d_perm, d_cand, d_cnt, d_map, d_status, d_flags, d_wlu;
+  DBuf d_piv, d_perm, d_cand, d_cnt, d_map, d_status, d_wlu;
   hipStream_t aux = nullptr;
   hipEvent_t lu_ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // the second-harmonic system's own streams, events, status and maps, so
@@ -165,13 +165,13 @@ struct omx_context {
   omx::LuProf prof{};
   double prof_gemm_ms = 0, prof_panel_ms = 0, prof_flop = 0, prof_total_ms = 0;
   int prof_launches = 0;
-  DBuf d_piv2, d_perm2, d_flags2;
+  DBuf d_piv2, d_perm2;
   DBuf d_dinv, d_dinv2, d_F64, d_F64sh;  // diagonal-block inverses; FP64 factors (the FP64 matrices stay intact)
   // wavelength sweeps: per-system buffers of one batch ([system][...])
   struct Sweep {
     DBuf S64, S32, V64, V32, F64, F64sh, eps, eps_sh, ov, incloc, Q, w, x, xint, xsh, xintsh, K, K1, cross, abscoef,
         abssh, absbes;
-    DBuf perm, perm2, flags, flags2, dinv, dinv2, map, status, basis, basisZ, r, w1, w2, gpart, part, hd, nrm, sc,
+    DBuf perm, perm2, dinv, dinv2, map, status, basis, basisZ, r, w1, w2, gpart, part, hd, nrm, sc,
         act, kz, omz, sys, map2, status2;
   } sw;
   // ACA-compressed operators S (0) and V (1) of OMX_SOLVER_ACA_GMRES
@@ -259,13 +259,12 @@ cplx C(const double *p, int i) { return omx::mk(p[2 * i], p[2 * i + 1]); }
 bool use32(int kind) { return kind == OMX_SOLVER_LU32_IR || kind == OMX_SOLVER_GMRES32_IR; }
 bool is_lu(int kind) { return kind == OMX_SOLVER_LU64 || kind == OMX_SOLVER_LU32_IR; }
 
-void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags, DBuf &dinv, bool second) {
+void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &dinv, bool second) {
   const int Np = c->Np;
   const int gmax = (Np + omx::kPanelRows - 1) / omx::kPanelRows + 1;
   piv.ensure(sizeof(int) * Np, &c->total);
   dinv.ensure(sizeof(cplx) * omx::dinv_elems(Np), &c->total);
   perm.ensure(sizeof(int) * Np, &c->total);
-  flags.ensure(sizeof(int) * (Np / omx::kNB + 1), &c->total);
   // tournament scratch: one half per system, as S and V may be factored
   // at the same time
   c->d_cand.ensure(sizeof(int) * 4 * gmax * omx::kNB, &c->total);
@@ -280,12 +279,10 @@ void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags
   }
   DBuf &map = second ? c->d_map2 : c->d_map;
   DBuf &status = second ? c->d_status2 : c->d_status;
-  HIPCHK(hipMemsetAsync(flags.p, 0, sizeof(int) * (Np / omx::kNB + 1), c->st));
   HIPCHK(hipMemsetAsync(status.p, 0, 64, c->st));
   w = omx::LuWork{};
   w.piv = piv.as<int>();
   w.perm = perm.as<int>();
-  w.flags = flags.as<int>();
   const int half = second ? 1 : 0;
   w.cand[0] = c->d_cand.as<int>() + (2 * half) * gmax * omx::kNB;
   w.cand[1] = c->d_cand.as<int>() + (2 * half + 1) * gmax * omx::kNB;
@@ -305,7 +302,6 @@ void alloc_lu(omx_context *c, omx::LuWork &w, DBuf &piv, DBuf &perm, DBuf &flags
   w.ev_bulk = ev[2];
   w.ev_pan = ev[3];
   w.gmax = gmax;
-  w.epoch = 0;
 }
 
 // (Re)allocates every size-dependent buffer.
@@ -383,8 +379,8 @@ void alloc_matrices(omx_context *c) {
   } else {
     c->ws_in.release(T);
   }
-  alloc_lu(c, c->lu, c->d_piv, c->d_perm, c->d_flags, c->d_dinv, false);
-  if(c->sh) alloc_lu(c, c->lu_sh, c->d_piv2, c->d_perm2, c->d_flags2, c->d_dinv2, true);
+  alloc_lu(c, c->lu, c->d_piv, c->d_perm, c->d_dinv, false);
+  if(c->sh) alloc_lu(c, c->lu_sh, c->d_piv2, c->d_perm2, c->d_dinv2, true);
 }
 
 float elapsed(omx_context *c, int a, int b) {
@@ -1376,7 +1372,6 @@ void sweep_alloc(omx_context *c, int nb, bool f64, bool sh) {
   w.abssh.ensure(sizeof(double) * P * nb, T);
   w.absbes.ensure(sizeof(cplx) * P * omx::kAbsBes * nb, T);
   for(DBuf *b : {&w.perm, &w.perm2}) b->ensure(sizeof(int) * Np * nb, T);
-  for(DBuf *b : {&w.flags, &w.flags2}) b->ensure(sizeof(int) * (Np / omx::kNB + 1) * nb, T);
   for(DBuf *b : {&w.dinv, &w.dinv2}) b->ensure(sizeof(cplx) * omx::dinv_elems(Np) * nb, T);
   w.map.ensure(sizeof(int) * 3 * (Np + 2 * omx::kNB) * nb, T);
   w.status.ensure(sizeof(int) * 4 * nb, T);
@@ -1403,7 +1398,6 @@ omx::LuWork sweep_lu(omx_context *c, int nb, bool second) {
   omx::LuWork lw{};
   lw.perm = (second ? w.perm2 : w.perm).as<int>();
   lw.piv = lw.perm;  // unused by block pivoting
-  lw.flags = (second ? w.flags2 : w.flags).as<int>();
   lw.dinv = (second ? w.dinv2 : w.dinv).p;
   const int ms = Np + 2 * omx::kNB;
   int *map = (second ? w.map2 : w.map).as<int>();
@@ -1420,7 +1414,6 @@ omx::LuWork sweep_lu(omx_context *c, int nb, bool second) {
   lw.ev_bulk = ev[2];
   lw.ev_pan = ev[3];
   lw.gmax = 0;
-  lw.epoch = 0;
   lw.pivot = omx::kPivotBlock;
   lw.split = c->lu32_gemm == OMX_GEMM_BF16X3;
   lw.nb = nb;
@@ -1430,7 +1423,6 @@ omx::LuWork sweep_lu(omx_context *c, int nb, bool second) {
   lw.bs.perm = Np;
   lw.bs.map = ms;
   lw.bs.cnt = ms;
-  lw.bs.flags = Np / omx::kNB + 1;
   lw.bs.status = 4;
   return lw;
 }
@@ -1484,8 +1476,8 @@ double time_median(hipStream_t st, const std::function<void()> &f, int reps = 5)
 
 omx::LuWork sweep_lu(omx_context *c, int nb, bool second);
 
-// The tuned choice for this batch (tuning it first if needed); must run
-// before the chunk clears its solve flags.
+// The tuned choice for this batch (tuning it first if needed); runs on the
+// chunk's buffers before the chunk fills them.
 TuneChoice sweep_tune(omx_context *c, int nb, bool f64) {
   const auto key = std::make_tuple(c->device, c->Np, nb, f64 ? 1 : 0);
   {
@@ -1516,7 +1508,6 @@ TuneChoice sweep_tune(omx_context *c, int nb, bool f64) {
     for(int i = 0; i < Np; ++i) ident[(size_t)z * Np + i] = i;
   HIPCHK(hipMemcpyAsync(w.perm.p, ident.data(), sizeof(int) * ident.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(w.dinv.p, 0, w.dinv.bytes, st));
-  HIPCHK(hipMemsetAsync(w.flags.p, 0, w.flags.bytes, st));
   HIPCHK(hipMemsetAsync(w.r.p, 0, w.r.bytes, st));
   omx::LuWork lw = sweep_lu(c, nb, false);
   for(int v = 0; v < omx::kTrsvVariants; ++v) {
@@ -1540,7 +1531,7 @@ TuneChoice sweep_tune(omx_context *c, int nb, bool f64) {
 
 // The single-system LU solves (omx_update / omx_solve) tune the same way,
 // with batch 1 and the context's own buffers (before the assembly fills
-// them); the timing launches advance the factorisation's solve epochs.
+// them).
 TuneChoice single_tune(omx_context *c, bool f64) {
   const auto key = std::make_tuple(c->device, c->Np, 1, f64 ? 1 : 0);
   {
@@ -1609,8 +1600,6 @@ void sweep_chunk(omx_context *c, int nb, const double *lam, const cplx *eps, con
   hipStream_t st = c->st;
   HIPCHK(hipMemcpyAsync(w.eps.p, eps, sizeof(cplx) * P * nb, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(w.eps_sh.p, eps_sh, sizeof(cplx) * P * nb, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(w.flags.p, 0, w.flags.bytes, st));
-  HIPCHK(hipMemsetAsync(w.flags2.p, 0, w.flags2.bytes, st));
   HIPCHK(hipMemsetAsync(w.status.p, 0, w.status.bytes, st));
   if(sh) HIPCHK(hipMemsetAsync(w.status2.p, 0, w.status2.bytes, st));
   // the lockstep refinement leaves the vectors of systems that dropped out
@@ -1919,8 +1908,7 @@ struct NoAllocLayout {
   static constexpr int Np = kNoAllocMaxN;
   static constexpr int gmax = Np / omx::kPanelRows + 1;
   static constexpr size_t status = 0;  // 16 ints
-  static constexpr size_t flags = status + align256(16 * sizeof(int));  // Np / 64 + 1 ints
-  static constexpr size_t piv = flags + align256((Np / omx::kNB + 1) * sizeof(int));  // Np ints
+  static constexpr size_t piv = status + align256(16 * sizeof(int));  // Np ints
   static constexpr size_t perm = piv + align256(Np * sizeof(int));  // Np ints
   static constexpr size_t cand = perm + align256(Np * sizeof(int));  // 2 x gmax x 64 ints
   static constexpr size_t cnt = cand + align256(2 * gmax * omx::kNB * sizeof(int));  // 2 x gmax ints
@@ -2015,15 +2003,15 @@ void omx_destroy(omx_context *c) {
                   &c->d_sph, &c->d_ov, &c->d_inc, &c->d_incloc, &c->d_Q, &c->d_w, &c->d_coef, &c->d_S64,
                   &c->d_S32, &c->d_V64, &c->d_V32, &c->d_x, &c->d_xsh, &c->d_xint, &c->d_xintsh, &c->d_K,
                   &c->d_K1, &c->d_cross, &c->d_cg, &c->d_piv, &c->d_perm, &c->d_cand, &c->d_cnt, &c->d_map,
-                  &c->d_wlu, &c->d_status, &c->d_flags, &c->d_basis, &c->d_basisZ, &c->d_w1, &c->d_w2, &c->d_r,
-                  &c->d_part, &c->d_h, &c->d_nrm, &c->d_gpart, &c->d_piv2, &c->d_perm2, &c->d_flags2,
+                  &c->d_wlu, &c->d_status, &c->d_basis, &c->d_basisZ, &c->d_w1, &c->d_w2, &c->d_r,
+                  &c->d_part, &c->d_h, &c->d_nrm, &c->d_gpart, &c->d_piv2, &c->d_perm2,
                   &c->d_dinv, &c->d_dinv2, &c->d_F64, &c->d_F64sh, &c->d_abscoef, &c->d_abssh, &c->d_absbes,
                   &c->d_fpts, &c->d_fout};
   for(DBuf *b : bufs) b->release(&c->total);
   auto &w = c->sw;
   DBuf *sbufs[] = {&w.S64, &w.S32, &w.V64, &w.V32, &w.F64, &w.F64sh, &w.eps, &w.eps_sh, &w.ov, &w.incloc, &w.Q,
-                   &w.w, &w.x, &w.xint, &w.xsh, &w.xintsh, &w.K, &w.K1, &w.cross, &w.perm, &w.perm2, &w.flags,
-                   &w.flags2, &w.dinv, &w.dinv2, &w.map, &w.status, &w.basis, &w.basisZ, &w.r, &w.w1, &w.w2, &w.gpart,
+                   &w.w, &w.x, &w.xint, &w.xsh, &w.xintsh, &w.K, &w.K1, &w.cross, &w.perm, &w.perm2,
+                   &w.dinv, &w.dinv2, &w.map, &w.status, &w.basis, &w.basisZ, &w.r, &w.w1, &w.w2, &w.gpart,
                    &w.part, &w.hd, &w.nrm, &w.sc, &w.act, &w.abscoef, &w.abssh, &w.absbes, &w.kz, &w.omz};
   for(DBuf *b : sbufs) b->release(&c->total);
   c->aca[0].release(&c->total);
@@ -2459,7 +2447,6 @@ void group_alloc(omx_context *L, int nb, bool f64, bool sh) {
   }
   for(DBuf *b : {&w.Q, &w.x, &w.xsh, &w.K, &w.r, &w.w1, &w.w2}) b->ensure(sizeof(cplx) * Np * nb, T);
   for(DBuf *b : {&w.perm, &w.perm2}) b->ensure(sizeof(int) * Np * nb, T);
-  for(DBuf *b : {&w.flags, &w.flags2}) b->ensure(sizeof(int) * (Np / omx::kNB + 1) * nb, T);
   for(DBuf *b : {&w.dinv, &w.dinv2}) b->ensure(sizeof(cplx) * omx::dinv_elems(Np) * nb, T);
   for(DBuf *b : {&w.map, &w.map2}) b->ensure(sizeof(int) * 3 * (Np + 2 * omx::kNB) * nb, T);
   for(DBuf *b : {&w.status, &w.status2}) b->ensure(sizeof(int) * 4 * nb, T);
@@ -2525,7 +2512,7 @@ int omx_solve_group(omx_context *L, int n, omx_context *const *mem) {
     const TuneChoice tune = L->autotune ? sweep_tune(L, nb, f64) : TuneChoice{L->lu32_gemm == OMX_GEMM_BF16X3, 0};
     auto &w = L->sw;
     hipStream_t st = L->st;
-    for(DBuf *b : {&w.flags, &w.flags2, &w.status, &w.status2, &w.basisZ, &w.w1}) HIPCHK(hipMemsetAsync(b->p, 0, b->bytes, st));
+    for(DBuf *b : {&w.status, &w.status2, &w.basisZ, &w.w1}) HIPCHK(hipMemsetAsync(b->p, 0, b->bytes, st));
     HIPCHK(hipMemsetAsync(L->d_status.p, 0, 16, st));
     // the members' matrices and right-hand sides, bordered by the identity
     cplx *S64 = w.S64.as<cplx>(), *V64 = sh ? w.V64.as<cplx>() : nullptr;
@@ -3095,7 +3082,6 @@ int omx_dense_solve_noalloc(omx_context *c, int N, double *A, size_t lda, const 
     auto ip = [&](size_t off) { return reinterpret_cast<int *>(base + off); };
     omx::LuWork w{};
     w.status = ip(L::status);
-    w.flags = ip(L::flags);
     w.piv = ip(L::piv);
     w.perm = ip(L::perm);
     w.cand[0] = ip(L::cand);
@@ -3113,12 +3099,10 @@ int omx_dense_solve_noalloc(omx_context *c, int N, double *A, size_t lda, const 
     w.ev_bulk = c->lu_ev[2];
     w.ev_pan = c->lu_ev[3];
     w.gmax = L::gmax;
-    w.epoch = 0;
     w.pivot = omx::kPivotPartial;
     w.nb = 1;
     cplx *rb = reinterpret_cast<cplx *>(base + L::rhs), *xs = reinterpret_cast<cplx *>(base + L::x);
-    // the array keeps the last call's contents: the status words and the
-    // triangular solves' flags (compared against epochs 1 and 2 below) start at zero
+    // the array keeps the last call's contents: the status words start at zero
     const int nz = (int)((L::piv - L::status) / sizeof(int));
     hipLaunchKernelGGL(zero_ints_kernel, dim3((nz + 255) / 256), dim3(256), 0, st, ip(L::status), nz);
     hipLaunchKernelGGL(stage_rhs_kernel, dim3((Np + 255) / 256), dim3(256), 0, st, reinterpret_cast<const cplx *>(b),

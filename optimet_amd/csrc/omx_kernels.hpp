@@ -158,7 +158,6 @@ struct BatchStrides {
   int perm = 0;    // permutations
   int map = 0;     // gather maps (mdst, msrc)
   int cnt = 0;     // map counts
-  int flags = 0;   // triangular-solve flags
   int status = 0;  // status words
 };
 
@@ -171,13 +170,11 @@ struct LuWork {
   int *mdst, *msrc;     // gather maps: tournament 2 NB entries at the start; block
   int *mcnt;            // pivoting NB entries per panel (panel k at k NB, count mcnt[k])
   int *status;          // [0]: singular pivot column+1, [1]: wait timeout
-  int *flags;           // Np / NB triangular-solve flags
   void *wlu;            // NB x NB complex: the tournament's selected-block factors
   void *dinv;           // per panel [L11^{-1} | U11^{-1}], 2 NB^2 elements of T, column-major
   hipStream_t aux;      // look-ahead stream of the panel pivoting
   hipEvent_t ev_next, ev_piv, ev_bulk, ev_pan;
   int gmax;
-  int epoch;            // last flag value used by a triangular solve
   int pivot;            // kPivotBlock or kPivotPartial
   int nb = 1;           // systems in the batch (block pivoting only when > 1)
   BatchStrides bs;      // their strides; the pointers above are system 0's

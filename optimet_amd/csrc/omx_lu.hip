@@ -26,8 +26,9 @@
 //                  operands split in three bf16 terms, six products per
 //                  real product, FP32 accumulation (FP32-accurate)
 //   trsv_kernel    sync-free blocked triangular solves: workgroup k owns
-//                  block row k, consumes earlier blocks as their flags are
-//                  published and applies its stored diagonal-block inverse
+//                  block row k, consumes earlier blocks as they are
+//                  published (tagged words) and applies its stored
+//                  diagonal-block inverse
 // Look-ahead: the pivoting of panel k+1 runs on a second stream while the
 // bulk of panel k's trailing update runs on the first.
 // This replaces the ScaLAPACK pzgesv path (scalapack/LinearSystemSolver.hpp:84-91)
@@ -1471,18 +1472,31 @@ int lu_factor(T *A, int Np, size_t lda, LuWork &w, hipStream_t st, LuProf *prof)
 }
 
 // ------------------------------------------------------------------ solves
+// Publication tags of the triangular solves: the low 4 bits of both 64-bit
+// words of a published x entry hold 1 (the lower solve) or 2 (the upper
+// solve), which changes the value by at most 2^-48 of itself.  The tags are
+// fixed, so results do not depend on the history of launches.
+constexpr unsigned long long kTagMask = 0xFull;
+__device__ inline double with_tag(double v, unsigned tg) {
+  return __longlong_as_double((long long)(((unsigned long long)__double_as_longlong(v) & ~kTagMask) | tg));
+}
+// The permuted right-hand side, with the tag bits cleared: a word the
+// triangular solves have not yet published can never carry their tag.
 __global__ void gather_kernel(const cplx *b, const int *perm, cplx *y, int Np, BatchStrides bs, const int *sys) {
   const size_t z = sys ? sys[blockIdx.z] : blockIdx.z;
   b += z * bs.vec;
   y += z * bs.vec;
   perm += z * bs.perm;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if(i < Np) y[i] = b[perm[i]];
+  if(i < Np) {
+    const cplx v = b[perm[i]];
+    y[i] = mk(with_tag(v.x, 0u), with_tag(v.y, 0u));
+  }
 }
 
 // Sync-free blocked triangular solve.  Workgroup K owns SB consecutive
 // 64-row blocks (a super-block).  Far phase: it consumes the blocks x_b of
-// earlier super-blocks as their flags are published and accumulates
+// earlier super-blocks as they are published and accumulates
 // L[super-block rows, b] x_b, with the next 64 x 16 tile slice of each thread
 // prefetched into registers while the current one is multiplied.  Near phase: the
 // super-block's own blocks are solved in sequence inside the workgroup -- the
@@ -1494,13 +1508,16 @@ __global__ void gather_kernel(const cplx *b, const int *perm, cplx *y, int Np, B
 // r = t % 64 and columns 16 (t / 64) .. 16 (t / 64) + 15 of every tile.
 //
 // Publication: a solved block is written with device-coherent (agent-scope)
-// stores, the workgroup waits for their completion, and one thread then
-// stores the block's flag, also device-coherent; consumers poll the flags
-// and read x with device-coherent loads, so neither side writes back or
-// invalidates the L2.
+// 64-bit stores whose low bits carry the solve's tag, and nothing else: a
+// consumer polls the block's words themselves with device-coherent loads
+// until every word carries the tag, so it needs no flag, no ordering between
+// stores, and no second read.  Each 64-bit word is single-copy atomic, and
+// the words it replaces carry another tag (0 in the right-hand side the
+// gather wrote, 1 from the lower solve before the upper one), so a tagged
+// word is always the new value.  Neither side writes back or invalidates the L2.
 template <typename T, bool LOWER, int SB, int PRE>
 __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size_t lda, int Np,
-                                                   const T *__restrict__ dinv, cplx *x, int *flags, int epoch,
+                                                   const T *__restrict__ dinv, cplx *x,
                                                    int *status, BatchStrides bs, const int *sys) {
   constexpr int NB = kNB;
   constexpr int NI = SB * (SB - 1) / 2;
@@ -1508,7 +1525,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
   A += zs * bs.a;
   dinv += zs * bs.inv;
   x += zs * bs.vec;
-  flags += zs * bs.flags;
+  constexpr unsigned tg = LOWER ? 1u : 2u;
   const int nblk = Np / NB, nsb = (nblk + SB - 1) / SB;
   const int K = LOWER ? blockIdx.x : nsb - 1 - blockIdx.x;
   const int kb0 = K * SB, nb = min(SB, nblk - kb0);
@@ -1562,36 +1579,45 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
   T cur[16], nxt[16];
   if(nfar > 0) tile(a_base(kb0, fblk(0)), lda, cur);
   for(int j0 = 0; j0 < nfar;) {
-    // how many of the next CH far blocks are published (wave 0, one flag per lane)
+    // the next far blocks, as far as they are published (wave 0, one entry
+    // per lane): each poll reads a whole block and keeps it when every word
+    // carries the tag; the blocks after a published one are tried at once
     if(t < 64) {
       unsigned spins = 0;
       int nr = 0;
-      while(true) {
-        const bool ok = t < CH && j0 + t < nfar &&
-                        __hip_atomic_load(&flags[fblk(j0 + t)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
-        const unsigned long long m = __ballot(ok);
-        nr = __builtin_ctzll(~m);  // leading run of published blocks
-        if(nr > 0 || __hip_atomic_load(&status[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      bool drain = false;
+      while(nr < CH && j0 + nr < nfar) {
+        const unsigned long long *src =
+            reinterpret_cast<const unsigned long long *>(x + (size_t)fblk(j0 + nr) * NB + t);
+        const unsigned long long re = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long im = __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool ok = (re & kTagMask) == tg && (im & kTagMask) == tg;
+        if(__ballot(ok) == ~0ull) {
+          xs[nr * NB + t] = mk(__longlong_as_double((long long)re), __longlong_as_double((long long)im));
+          ++nr;
+          continue;
+        }
+        if(nr > 0) break;
+        if(__builtin_amdgcn_readfirstlane(
+               __hip_atomic_load(&status[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0) {
+          drain = true;
+          break;
+        }
         if(++spins > (1u << 24)) {
           if(t == 0) __hip_atomic_store(&status[1], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          drain = true;
           break;
         }
         __builtin_amdgcn_s_sleep(1);
       }
-      if(t == 0) nready_s = nr > 0 ? nr : nfar - j0;  // on a timeout: drain (the host reports the error)
+      if(drain) {  // a timed-out wait: finish with zeros (the host reports the error)
+        nr = min(CH, nfar - j0);
+        for(int c = 0; c < nr; ++c) xs[c * NB + t] = mk(0.0);
+      }
+      if(t == 0) nready_s = nr;
     }
     __syncthreads();
     const int nr = nready_s;
-    // those blocks of x were stored device-coherently before their flags:
-    // read them with device-coherent loads
-    for(int idx = t; idx < nr * NB; idx += 256) {
-      const unsigned long long *src =
-          reinterpret_cast<const unsigned long long *>(x + (size_t)fblk(j0 + idx / NB) * NB + (idx % NB));
-      unsigned long long re = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      unsigned long long im = __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      xs[idx] = mk(__longlong_as_double((long long)re), __longlong_as_double((long long)im));
-    }
-    __syncthreads();
     for(int c = 0; c < nr; ++c) {
       const int j = j0 + c, b = fblk(j);
 #pragma unroll
@@ -1686,29 +1712,16 @@ __global__ __launch_bounds__(256) void trsv_kernel(const T *__restrict__ A, size
       part[cq][r] = y;
       __syncthreads();
       if(t < NB) {
-        const cplx xk = part[0][t] + part[1][t] + part[2][t] + part[3][t];
+        const cplx sum = part[0][t] + part[1][t] + part[2][t] + part[3][t];
+        const cplx xk = mk(with_tag(sum.x, tg), with_tag(sum.y, tg));
         xsb[s][t] = xk;
         unsigned long long *dst = reinterpret_cast<unsigned long long *>(x + (size_t)k * NB + t);
         __hip_atomic_store(dst, (unsigned long long)__double_as_longlong(xk.x), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(dst + 1, (unsigned long long)__double_as_longlong(xk.y), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
-        // Publication order.  Wave 0 (threads 0..63) issues every x store of
-        // the block and, below, the flag store.  The x stores are agent-scope
-        // (sc1, written through to the device-coherent level), and a store's
-        // vmcnt slot is released only when that write is acknowledged there,
-        // so waiting for vmcnt(0) here makes x_k visible to every XCD before
-        // the flag can be issued.  A workgroup-scope release fence emits no
-        // such wait, and an agent-scope release would add an L2 writeback
-        // that these write-through stores do not need.  The memory clobber
-        // keeps the compiler from moving memory operations across the wait.
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       }
-      __syncthreads();
-      // Consumers poll the flag with agent-scope loads and read x_k with
-      // agent-scope loads issued only after the poll has returned the epoch
-      // (the ballot, branch and barrier depend on its value), so they see x_k.
-      if(t == 0) __hip_atomic_store(&flags[k], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();  // part and xs are rewritten by the next block
     }
   }
 }
@@ -1734,12 +1747,10 @@ static void trsv_pair(const T *A, int Np, size_t lda, LuWork &w, cplx *x, int nz
   constexpr int PRE = TrsvCfg<T>::PRE[V];
   const T *dinv = reinterpret_cast<const T *>(w.dinv);
   const int nblk = Np / kNB, nsb = (nblk + SB - 1) / SB;
-  int e1 = ++w.epoch;
   hipLaunchKernelGGL((trsv_kernel<T, true, SB, PRE>), dim3(nsb, 1, nz), dim3(256), 0, st, A, lda, Np, dinv, x,
-                     w.flags, e1, w.status, w.bs, w.sys);
-  int e2 = ++w.epoch;
+                     w.status, w.bs, w.sys);
   hipLaunchKernelGGL((trsv_kernel<T, false, SB, PRE>), dim3(nsb, 1, nz), dim3(256), 0, st, A, lda, Np, dinv, x,
-                     w.flags, e2, w.status, w.bs, w.sys);
+                     w.status, w.bs, w.sys);
 }
 
 template <typename T>

@@ -59,7 +59,7 @@ def rel(a, b):
 
 def test_noalloc_matches_dense_solve_twice_second_smaller():
     """Two calls in a row (N = 1000, then N = 300 in the same scratch, whose
-    flags and pivots still hold the first call's values): each x equals the
+    status words, pivots and staged vectors still hold the first call's values): each x equals the
     allocating entry point's FP64 partial-pivoting solution to 1e-12 (the
     same kernels run on the same data) and LAPACK's to 1e-10 (the FP64 bar)."""
     from optimet_amd import solver
