@@ -34,21 +34,21 @@ __global__ void vmul_ov_k(cplx *y, const cplx *x, const cplx *ov, int kind, int 
 
 // out[k] = sum_i conj(V[k][i]) w[i]; one partial per block, then reduced.
 constexpr int kRedBlocks = 128;
+// Partial sums of v_k^H w, one workgroup row per vector k (blockIdx.y): the
+// vectors' reductions run side by side instead of one after another.
 __global__ __launch_bounds__(256) void dotc_multi_k(const cplx *V, size_t ldv, int nv, const cplx *w, int n,
                                                     cplx *part) {
   __shared__ cplx red[256];
-  for(int k = 0; k < nv; ++k) {
-    cplx s = mk(0.0);
-    for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s = s + conj(V[k * ldv + i]) * w[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for(int st = 128; st > 0; st >>= 1) {
-      if(threadIdx.x < st) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + st];
-      __syncthreads();
-    }
-    if(threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = red[0];
+  const int k = blockIdx.y;
+  cplx s = mk(0.0);
+  for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s = s + conj(V[k * ldv + i]) * w[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for(int st = 128; st > 0; st >>= 1) {
+    if(threadIdx.x < st) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + st];
     __syncthreads();
   }
+  if(threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = red[0];
 }
 __global__ void reduce_rows_k(const cplx *part, int nb, int nv, cplx *out) {
   int k = blockIdx.x;
@@ -159,7 +159,7 @@ void vmul_ov(cplx *y, const cplx *x, const cplx *ov, int kind, int P, int nmax, 
 }
 void dotc_multi(const cplx *V, size_t ldv, int nv, const cplx *w, int n, cplx *part, cplx *out, hipStream_t st) {
   int nb = std::min(kRedBlocks, (n + 255) / 256);
-  hipLaunchKernelGGL(dotc_multi_k, dim3(nb), dim3(256), 0, st, V, ldv, nv, w, n, part);
+  hipLaunchKernelGGL(dotc_multi_k, dim3(nb, nv), dim3(256), 0, st, V, ldv, nv, w, n, part);
   hipLaunchKernelGGL(reduce_rows_k, dim3(nv), dim3(128), 0, st, part, nb, nv, out);
 }
 void axpy_multi(const cplx *V, size_t ldv, int nv, const cplx *h, cplx *w, int n, hipStream_t st) {
@@ -190,25 +190,23 @@ __global__ void vsub_b_k(cplx *zz, const cplx *x, const cplx *y, int n, size_t s
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i < n) zz[o + i] = x[o + i] - y[o + i];
 }
+// The batched form: system z = blockIdx.y, vector k = blockIdx.z.
 __global__ __launch_bounds__(256) void dotc_multi_b_k(const cplx *V, size_t ldv, size_t sV, int nv, const cplx *w,
                                                       size_t sv, int n, cplx *part) {
-  const int z = blockIdx.y;
+  const int z = blockIdx.y, k = blockIdx.z;
   V += z * sV;
   w += z * sv;
   part += (size_t)z * nv * gridDim.x;
   __shared__ cplx red[256];
-  for(int k = 0; k < nv; ++k) {
-    cplx s = mk(0.0);
-    for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s = s + conj(V[k * ldv + i]) * w[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for(int st = 128; st > 0; st >>= 1) {
-      if(threadIdx.x < st) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + st];
-      __syncthreads();
-    }
-    if(threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = red[0];
+  cplx s = mk(0.0);
+  for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s = s + conj(V[k * ldv + i]) * w[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for(int st = 128; st > 0; st >>= 1) {
+    if(threadIdx.x < st) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + st];
     __syncthreads();
   }
+  if(threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = red[0];
 }
 __global__ void reduce_rows_b_k(const cplx *part, int nb, int nv, cplx *out, int so) {
   const int z = blockIdx.y, k = blockIdx.x;
@@ -287,7 +285,7 @@ int red_blocks_b(int n, int nb) { return std::max(1, std::min(kRedBlocks / std::
 void dotc_multi_b(const cplx *V, size_t ldv, size_t sV, int nv, const cplx *w, size_t sv, int n, int nb, cplx *part,
                   cplx *out, int so, hipStream_t st) {
   const int rb = red_blocks_b(n, nb);
-  hipLaunchKernelGGL(dotc_multi_b_k, dim3(rb, nb), dim3(256), 0, st, V, ldv, sV, nv, w, sv, n, part);
+  hipLaunchKernelGGL(dotc_multi_b_k, dim3(rb, nb, nv), dim3(256), 0, st, V, ldv, sV, nv, w, sv, n, part);
   hipLaunchKernelGGL(reduce_rows_b_k, dim3(nv, nb), dim3(128), 0, st, part, rb, nv, out, so);
 }
 void axpy_multi_b(const cplx *V, size_t ldv, size_t sV, int nv, const cplx *h, int sh, cplx *w, size_t sv, int n,
